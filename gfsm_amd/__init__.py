@@ -1,0 +1,389 @@
+"""gfsm_amd -- MI355X-native forest-based subgraph matching (GFSM semantics).
+
+Python binding of ``libgfsm.so`` over its C ABI (``include/gfsm.h``).  The
+compute path is the HIP library; this module only marshals arrays and handles.
+Loading fails loudly when the library has not been built.
+
+    ctx = gfsm_amd.Context()
+    data = gfsm_amd.Graph.rmat(18, 8, nvlabels=16, mode=gfsm_amd.MODE_VLABEL)
+    query = gfsm_amd.Graph.walk_query(data, 8, seed=1)
+    m = ctx.match(data, query, digest=True)
+    print(m.count, m.digest)
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+__all__ = [
+    "GfsmError", "Graph", "Context", "Match", "Server", "lib", "library_path", "row_hash",
+    "MODE_REF", "MODE_VLABEL", "ROWS", "SORTED", "DIGEST", "PROFILE", "WALK_DENSE", "WALK_SPARSE",
+]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+
+MODE_REF = 0
+MODE_VLABEL = 1
+ROWS = 1
+SORTED = 2
+DIGEST = 4
+PROFILE = 8
+WALK_DENSE = 0
+WALK_SPARSE = 1
+ERR_PENDING = 6
+MAX_Q = 32
+
+
+class GfsmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"gfsm error {code}: {msg}")
+        self.code = code
+
+
+class MatchOpts(C.Structure):
+    _fields_ = [("flags", C.c_uint32), ("root_lo", C.c_uint32), ("root_hi", C.c_uint32),
+                ("tile", C.c_uint32), ("max_rows", C.c_uint64), ("mem_budget", C.c_uint64)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("q", C.c_uint32), ("launches", C.c_uint32), ("chunks", C.c_uint32),
+                ("overflows", C.c_uint32), ("roots", C.c_uint64),
+                ("level", C.c_uint64 * MAX_Q), ("cand", C.c_uint64 * MAX_Q),
+                ("checks", C.c_uint32 * MAX_Q), ("alg_bytes", C.c_uint64),
+                ("device_ms", C.c_double), ("kernel_ms", C.c_double), ("expand_ms", C.c_double),
+                ("expand_launches", C.c_uint32), ("pad", C.c_uint32), ("host_ms", C.c_double)]
+
+    def as_dict(self):
+        q = self.q
+        return {
+            "q": q, "launches": self.launches, "chunks": self.chunks, "overflows": self.overflows,
+            "roots": self.roots, "level": list(self.level[:q]), "cand": list(self.cand[:q]),
+            "checks": list(self.checks[:q]), "alg_bytes": self.alg_bytes,
+            "device_ms": self.device_ms, "kernel_ms": self.kernel_ms, "expand_ms": self.expand_ms,
+            "expand_launches": self.expand_launches, "host_ms": self.host_ms,
+        }
+
+
+class ServeOpts(C.Structure):
+    _fields_ = [("mem_budget", C.c_uint64), ("max_inflight", C.c_uint32), ("flags", C.c_uint32)]
+
+
+_LIB = None
+
+
+def library_path():
+    return os.path.join(_HERE, "libgfsm.so")
+
+
+def lib():
+    """The loaded libgfsm.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = library_path()
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with __graft_entry__.build() (make -C gfsm_amd)")
+    L = C.CDLL(path)
+    P, u32, u64, vp = C.POINTER, C.c_uint32, C.c_uint64, C.c_void_p
+    pu32 = P(u32)
+    sigs = {
+        "gfsm_open": [C.c_int, u64, P(vp)],
+        "gfsm_close": [vp],
+        "gfsm_graph_load_tve": [C.c_char_p, u32, P(vp)],
+        "gfsm_graph_from_edges": [u32, pu32, u64, pu32, pu32, pu32, u32, P(vp)],
+        "gfsm_graph_write_tve": [vp, C.c_char_p],
+        "gfsm_graph_edges": [vp, P(u32), P(u64), P(pu32), P(pu32), P(pu32), P(pu32)],
+        "gfsm_graph_layout": [vp, P(u32), P(u64), P(u64), P(pu32), P(pu32), P(pu32)],
+        "gfsm_graph_info": [vp, P(u32), P(u64), P(u32), P(u32)],
+        "gfsm_graph_upload": [vp, vp],
+        "gfsm_graph_free": [vp],
+        "gfsm_plan_query": [vp, P(u32), P(u32), pu32, pu32, pu32],
+        "gfsm_match": [vp, vp, vp, P(MatchOpts), P(vp)],
+        "gfsm_result_count": [vp, P(u64)],
+        "gfsm_result_rows": [vp, P(pu32), P(u32), P(u64)],
+        "gfsm_result_digest": [vp, P(u64)],
+        "gfsm_result_stats": [vp, P(Stats)],
+        "gfsm_result_status": [vp],
+        "gfsm_result_write_fs": [vp, C.c_char_p],
+        "gfsm_result_free": [vp],
+        "gfsm_row_hash": [pu32, u32, P(u64)],
+        "gfsm_match_batch": [vp, vp, P(vp), u32, P(MatchOpts), P(vp)],
+        "gfsm_server_start": [vp, vp, P(ServeOpts), P(vp)],
+        "gfsm_server_submit": [vp, vp, P(u64)],
+        "gfsm_server_poll": [vp, u64, P(vp)],
+        "gfsm_server_wait": [vp, u64, P(vp)],
+        "gfsm_server_times": [vp, u64, P(C.c_double)],
+        "gfsm_server_stop": [vp],
+        "gfsm_selftest_scan": [vp, pu32, u64, P(u64), pu32, P(u64)],
+        "gfsm_gen_er": [u32, C.c_double, u32, u32, u64, u32, P(vp)],
+        "gfsm_gen_rmat": [u32, u32, C.c_double, C.c_double, C.c_double, u32, C.c_double, u32, u64, u32, P(vp)],
+        "gfsm_gen_grid": [u32, u32, C.c_double, C.c_double, u64, u32, P(vp)],
+        "gfsm_gen_query_walk": [vp, u32, u32, u64, P(vp)],
+    }
+    for name, args in sigs.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = C.c_int32
+    for name in ("gfsm_close", "gfsm_graph_free", "gfsm_result_free", "gfsm_server_stop", "gfsm_row_hash"):
+        getattr(L, name).restype = None
+    L.gfsm_last_error.restype = C.c_char_p
+    L.gfsm_last_error.argtypes = []
+    L.gfsm_version.restype = C.c_char_p
+    _LIB = L
+    return L
+
+
+def _check(rc, what=""):
+    if rc != 0:
+        msg = lib().gfsm_last_error().decode(errors="replace")
+        raise GfsmError(rc, f"{what}: {msg}" if what else msg)
+
+
+def _u32(a):
+    a = np.ascontiguousarray(a, dtype=np.uint32)
+    return a, a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+def _view(ptr, n, dtype=np.uint32):
+    if n == 0 or not ptr:
+        return np.zeros(0, dtype=dtype)
+    return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True)
+
+
+def row_hash(row):
+    a, p = _u32(row)
+    h = (C.c_uint64 * 2)()
+    lib().gfsm_row_hash(p, len(a), h)
+    return int(h[0]), int(h[1])
+
+
+class Graph:
+    """A host graph in the relation-partitioned layout (uploaded on first use)."""
+
+    def __init__(self, handle):
+        self._h = C.c_void_p(handle) if not isinstance(handle, C.c_void_p) else handle
+
+    def __del__(self):
+        self.free()
+
+    def free(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().gfsm_graph_free(self._h)
+            self._h = C.c_void_p()
+
+    @classmethod
+    def _make(cls, fn, *args, what=""):
+        h = C.c_void_p()
+        _check(fn(*args, C.byref(h)), what)
+        return cls(h)
+
+    @classmethod
+    def load(cls, path, mode=MODE_REF):
+        return cls._make(lib().gfsm_graph_load_tve, str(path).encode(), mode, what=f"load {path}")
+
+    @classmethod
+    def from_edges(cls, nv, src, dst, elabel, vlabel=None, mode=MODE_REF):
+        s, ps = _u32(src)
+        d, pd = _u32(dst)
+        e, pe = _u32(elabel)
+        if vlabel is None:
+            pv = None
+        else:
+            v, pv = _u32(vlabel)
+        return cls._make(lib().gfsm_graph_from_edges, int(nv), pv, len(s), ps, pd, pe, mode, what="from_edges")
+
+    @classmethod
+    def er(cls, n, avg_degree, nvlabels=1, nelabels=1, seed=0, mode=MODE_REF):
+        return cls._make(lib().gfsm_gen_er, n, float(avg_degree), nvlabels, nelabels, seed, mode, what="gen_er")
+
+    @classmethod
+    def rmat(cls, scale, edge_factor, a=0.57, b=0.19, c=0.19, nvlabels=1, zipf_s=1.0, nelabels=1, seed=0,
+             mode=MODE_REF):
+        return cls._make(lib().gfsm_gen_rmat, scale, edge_factor, a, b, c, nvlabels, float(zipf_s), nelabels,
+                         seed, mode, what="gen_rmat")
+
+    @classmethod
+    def grid(cls, rows, cols, del_frac=0.0, diag_frac=0.0, seed=0, mode=MODE_REF):
+        return cls._make(lib().gfsm_gen_grid, rows, cols, float(del_frac), float(diag_frac), seed, mode,
+                         what="gen_grid")
+
+    @classmethod
+    def walk_query(cls, data, qsize, kind=WALK_DENSE, seed=0):
+        return cls._make(lib().gfsm_gen_query_walk, data._h, qsize, kind, seed, what="gen_query_walk")
+
+    def write(self, path):
+        _check(lib().gfsm_graph_write_tve(self._h, str(path).encode()), "write")
+
+    def info(self):
+        nv, ne, md, mode = C.c_uint32(), C.c_uint64(), C.c_uint32(), C.c_uint32()
+        _check(lib().gfsm_graph_info(self._h, C.byref(nv), C.byref(ne), C.byref(md), C.byref(mode)))
+        return {"V": nv.value, "entries": ne.value, "max_degree": md.value, "mode": mode.value}
+
+    @property
+    def V(self):
+        return self.info()["V"]
+
+    def edges(self):
+        """(V, src, dst, elabel, vlabel) copies of the input entries."""
+        nv, ne = C.c_uint32(), C.c_uint64()
+        ps = [C.POINTER(C.c_uint32)() for _ in range(4)]
+        _check(lib().gfsm_graph_edges(self._h, C.byref(nv), C.byref(ne), C.byref(ps[0]), C.byref(ps[1]),
+                                      C.byref(ps[2]), C.byref(ps[3])))
+        n = ne.value
+        return (nv.value, _view(ps[1], n), _view(ps[2], n), _view(ps[3], n), _view(ps[0], nv.value))
+
+    def layout(self):
+        """(dir_ptr, dir as (ndir+1, 2) {relation, begin}, nbr) copies of the layout."""
+        nv, nd, nn = C.c_uint32(), C.c_uint64(), C.c_uint64()
+        ps = [C.POINTER(C.c_uint32)() for _ in range(3)]
+        _check(lib().gfsm_graph_layout(self._h, C.byref(nv), C.byref(nd), C.byref(nn), C.byref(ps[0]),
+                                       C.byref(ps[1]), C.byref(ps[2])))
+        return (_view(ps[0], nv.value + 1), _view(ps[1], 2 * (nd.value + 1)).reshape(-1, 2),
+                _view(ps[2], nn.value))
+
+    def plan(self):
+        """Reference-layout plan: (col, header[3q], requirements[2*maxval*q])."""
+        q, mv = C.c_uint32(), C.c_uint32()
+        _check(lib().gfsm_plan_query(self._h, C.byref(q), C.byref(mv), None, None, None), "plan")
+        col = np.zeros(q.value, np.uint32)
+        hdr = np.zeros(3 * q.value, np.uint32)
+        req = np.zeros(max(1, 2 * mv.value * q.value), np.uint32)
+        P = C.POINTER(C.c_uint32)
+        _check(lib().gfsm_plan_query(self._h, C.byref(q), C.byref(mv), col.ctypes.data_as(P),
+                                     hdr.ctypes.data_as(P), req.ctypes.data_as(P)), "plan")
+        return col, hdr, req[: 2 * mv.value * q.value]
+
+
+class Match:
+    """Result of one match: count, digest, optional rows and kernel statistics."""
+
+    def __init__(self, handle):
+        self._h = handle
+        L = lib()
+        n = C.c_uint64()
+        _check(L.gfsm_result_count(handle, C.byref(n)))
+        self.count = n.value
+        dg = (C.c_uint64 * 2)()
+        _check(L.gfsm_result_digest(handle, dg))
+        self.digest = (int(dg[0]), int(dg[1]))
+        st = Stats()
+        _check(L.gfsm_result_stats(handle, C.byref(st)))
+        self.stats = st.as_dict()
+        self.status = L.gfsm_result_status(handle)
+        p = C.POINTER(C.c_uint32)()
+        w, nr = C.c_uint32(), C.c_uint64()
+        _check(L.gfsm_result_rows(handle, C.byref(p), C.byref(w), C.byref(nr)))
+        self.width = w.value
+        self.rows = _view(p, nr.value * w.value).reshape(nr.value, w.value) if nr.value else None
+
+    def write_fs(self, path):
+        _check(lib().gfsm_result_write_fs(self._h, str(path).encode()), "write_fs")
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().gfsm_result_free(self._h)
+            self._h = C.c_void_p()
+
+
+def _opts(rows=False, sorted=False, digest=False, profile=False, root_lo=0, root_hi=0, max_rows=0,
+          mem_budget=0):
+    flags = (ROWS if rows else 0) | (SORTED if sorted else 0) | (DIGEST if digest else 0) | \
+        (PROFILE if profile else 0)
+    return MatchOpts(flags, root_lo, root_hi, 0, max_rows, mem_budget)
+
+
+class Context:
+    """A device context: one HIP stream, its scan workspace and a level arena."""
+
+    def __init__(self, device=0, mem_budget=0):
+        self._h = C.c_void_p()
+        _check(lib().gfsm_open(device, mem_budget, C.byref(self._h)), "open")
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().gfsm_close(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def selftest_scan(self, vals):
+        """Chained wave64 scan on the device: (exclusive prefix u64, compacted indices, (nonzero, sum))."""
+        v, pv = _u32(vals)
+        n = len(v)
+        pre = np.zeros(max(n, 1), np.uint64)
+        idx = np.zeros(max(n, 1), np.uint32)
+        tot = (C.c_uint64 * 2)()
+        _check(lib().gfsm_selftest_scan(self._h, pv, n, pre.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                        idx.ctypes.data_as(C.POINTER(C.c_uint32)), tot), "selftest_scan")
+        return pre[:n], idx[: int(tot[0])], (int(tot[0]), int(tot[1]))
+
+    def upload(self, g: Graph):
+        _check(lib().gfsm_graph_upload(self._h, g._h), "upload")
+
+    def match(self, data: Graph, query: Graph, **kw) -> Match:
+        o = _opts(**kw)
+        r = C.c_void_p()
+        _check(lib().gfsm_match(self._h, data._h, query._h, C.byref(o), C.byref(r)), "match")
+        return Match(r)
+
+    def match_batch(self, data: Graph, queries, **kw):
+        o = _opts(**kw)
+        n = len(queries)
+        hs = (C.c_void_p * n)(*[q._h.value for q in queries])
+        rs = (C.c_void_p * n)()
+        _check(lib().gfsm_match_batch(self._h, data._h, hs, n, C.byref(o), rs), "match_batch")
+        return [Match(C.c_void_p(rs[i])) for i in range(n)]
+
+
+class Server:
+    """Serving loop over a resident data graph (submit / poll / wait)."""
+
+    def __init__(self, ctx: Context, data: Graph, mem_budget=0, max_inflight=0, digest=False):
+        self._h = C.c_void_p()
+        self._ctx, self._data = ctx, data
+        o = ServeOpts(mem_budget, max_inflight, DIGEST if digest else 0)
+        _check(lib().gfsm_server_start(ctx._h, data._h, C.byref(o), C.byref(self._h)), "server_start")
+        self._queries = {}
+
+    def submit(self, query: Graph) -> int:
+        t = C.c_uint64()
+        _check(lib().gfsm_server_submit(self._h, query._h, C.byref(t)), "submit")
+        self._queries[t.value] = query  # keep the query alive while in flight
+        return t.value
+
+    def poll(self, ticket):
+        r = C.c_void_p()
+        rc = lib().gfsm_server_poll(self._h, ticket, C.byref(r))
+        if rc == ERR_PENDING:
+            return None
+        _check(rc, "poll")
+        self._queries.pop(ticket, None)
+        return Match(r)
+
+    def wait(self, ticket) -> Match:
+        r = C.c_void_p()
+        _check(lib().gfsm_server_wait(self._h, ticket, C.byref(r)), "wait")
+        self._queries.pop(ticket, None)
+        return Match(r)
+
+    def times(self, ticket):
+        t = (C.c_double * 3)()
+        _check(lib().gfsm_server_times(self._h, ticket, t), "times")
+        return tuple(t)
+
+    def stop(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().gfsm_server_stop(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        self.stop()

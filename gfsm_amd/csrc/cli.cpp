@@ -1,0 +1,111 @@
+// cli.cpp -- `GFSM <query.g> <data.g>`: the reference's command line
+// (main.cu:56-63) over libgfsm.  Prints the reference's CSV row
+// (mainscripts/runLabel.sh:19 column order) and exits with its codes
+// (232/233/234 for input errors, 1 for query errors).
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+
+#include "gfsm.h"
+
+static double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+static std::string base_name(const std::string &p)
+{
+    size_t k = p.find_last_of('/');
+    return k == std::string::npos ? p : p.substr(k + 1);
+}
+
+static int die(gfsm_status st)
+{
+    std::fprintf(stderr, "\nGFSM error %d: %s\n", st, gfsm_last_error());
+    return st;
+}
+
+int main(int argc, char **argv)
+{
+    if (argc < 3) {
+        std::fprintf(stderr,
+                     "usage: %s <query.g> <data.g> [--mode ref|vlabel] [--out output.fs] [--sorted]\n"
+                     "          [--mem-budget BYTES] [--device N] [--digest]\n",
+                     argv[0]);
+        return 2;
+    }
+    uint32_t mode = GFSM_MODE_REF;
+    const char *out = nullptr;
+    uint64_t budget = 0;
+    int device = 0;
+    uint32_t flags = 0;
+    for (int i = 3; i < argc; i++) {
+        if (!std::strcmp(argv[i], "--mode") && i + 1 < argc) {
+            mode = !std::strcmp(argv[++i], "vlabel") ? GFSM_MODE_VLABEL : GFSM_MODE_REF;
+        } else if (!std::strcmp(argv[i], "--out") && i + 1 < argc) {
+            out = argv[++i];
+        } else if (!std::strcmp(argv[i], "--mem-budget") && i + 1 < argc) {
+            budget = std::strtoull(argv[++i], nullptr, 10);
+        } else if (!std::strcmp(argv[i], "--device") && i + 1 < argc) {
+            device = std::atoi(argv[++i]);
+        } else if (!std::strcmp(argv[i], "--digest")) {
+            flags |= GFSM_DIGEST;
+        } else if (!std::strcmp(argv[i], "--sorted")) {
+            flags |= GFSM_SORTED;
+        }
+    }
+    if (out) flags |= GFSM_ROWS;
+    std::printf("\n%s,%s,", base_name(argv[1]).c_str(), base_name(argv[2]).c_str());
+    const double t_start = now_s();
+    gfsm_ctx *ctx = nullptr;
+    double t = now_s();
+    gfsm_status st = gfsm_open(device, budget, &ctx);
+    if (st) return die(st);
+    std::printf("%fs,", now_s() - t);  // Preallocate Time
+
+    gfsm_graph *q = nullptr, *d = nullptr;
+    t = now_s();
+    st = gfsm_graph_load_tve(argv[1], mode, &q);
+    if (st) return die(st);
+    std::printf("%fs,", now_s() - t);  // Query Parse Time
+    t = now_s();
+    st = gfsm_graph_load_tve(argv[2], mode, &d);
+    if (st) return die(st);
+    std::printf("%fs,", now_s() - t);  // Data Parse Time
+
+    gfsm_match_opts o{};
+    o.flags = flags;
+    gfsm_result *r = nullptr;
+    const double t_gpu = now_s();
+    st = gfsm_match(ctx, d, q, &o, &r);
+    if (st) return die(st);
+    const double gpu_s = now_s() - t_gpu;
+    gfsm_stats s;
+    gfsm_result_stats(r, &s);
+    uint64_t count = 0;
+    gfsm_result_count(r, &count);
+    std::printf("%fs,", s.device_ms / 1e3);  // Kernel Time
+    std::printf("n/a,n/a,");                  // Reallocate / ReKernel: no retry exists
+    std::printf("%llu,", (unsigned long long)count);
+    t = now_s();
+    if (out) {
+        st = gfsm_result_write_fs(r, out);
+        if (st) return die(st);
+    }
+    std::printf("%fs,", now_s() - t);  // Solution Write Time
+    std::printf("%fs,", gpu_s);        // GPU Time
+    if (flags & GFSM_DIGEST) {
+        uint64_t dg[2];
+        gfsm_result_digest(r, dg);
+        std::fprintf(stderr, "digest=%016llx%016llx\n", (unsigned long long)dg[0], (unsigned long long)dg[1]);
+    }
+    gfsm_result_free(r);
+    gfsm_graph_free(q);
+    gfsm_graph_free(d);
+    gfsm_close(ctx);
+    std::printf("%fs,", now_s() - t_start);  // End to End Time
+    std::printf("\n");
+    return 0;
+}

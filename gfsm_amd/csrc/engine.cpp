@@ -1,0 +1,597 @@
+// engine.cpp -- contexts, resident graphs and the plain (single-query)
+// matcher driver.
+//
+// Replaces HnSetup::preinit / gpuLaunch and the device-side orchestrator
+// hnSolve (HnGPU.cu:847-1063, 455-669).  The reference drives every depth
+// from one device thread with dynamic parallelism and, when a level would
+// overflow its fixed buffers, either batches on the device (HnGPU.cu:529-630)
+// or frees everything and reruns at twice the memory (HnGPU.cu:957-992).
+// Here the host drives the depths, and a level that does not fit its buffer
+// is expanded in candidate-range chunks, depth first: each chunk's subtree
+// runs to the leaves before the next chunk is generated, so memory is bounded
+// by the budget whatever the output size, and the result does not depend on
+// how the work was chunked.
+#include "engine.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <functional>
+
+namespace gfsm {
+
+static std::mutex g_dev_mu;  // guards HostGraph::dev and Ctx::graphs
+
+static double now_ms()
+{
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+#define HIP_TRY(expr)                                                    \
+    do {                                                                 \
+        hipError_t e_ = (expr);                                          \
+        if (e_ != hipSuccess) return hip_status(e_, #expr);              \
+    } while (0)
+
+Status Lane::init(uint64_t max_tiles)
+{
+    HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    ws.max_tiles = max_tiles;
+    HIP_TRY(hipMalloc(&ws.flag, max_tiles * sizeof(uint32_t)));
+    HIP_TRY(hipMalloc(&ws.agg, max_tiles * sizeof(Pair64)));
+    HIP_TRY(hipMalloc(&ws.inc, max_tiles * sizeof(Pair64)));
+    HIP_TRY(hipMalloc(&ws.ctl, 4 * sizeof(uint32_t)));
+    HIP_TRY(hipMemset(ws.flag, 0, max_tiles * sizeof(uint32_t)));
+    HIP_TRY(hipMemset(ws.ctl, 0, 4 * sizeof(uint32_t)));
+    HIP_TRY(hipMalloc(&tot_d, sizeof(Totals)));
+    HIP_TRY(hipMemset(tot_d, 0, sizeof(Totals)));
+    HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&tot_h), sizeof(Totals), hipHostMallocDefault));
+    std::memset(tot_h, 0, sizeof(Totals));
+    for (int i = 0; i < 2; i++) {
+        HIP_TRY(hipEventCreate(&ev[i]));
+        HIP_TRY(hipEventCreate(&kev[i]));
+    }
+    return Status::ok();
+}
+
+void Lane::destroy()
+{
+    if (stream) (void)hipStreamSynchronize(stream);
+    (void)hipFree(ws.flag);
+    (void)hipFree(ws.agg);
+    (void)hipFree(ws.inc);
+    (void)hipFree(ws.ctl);
+    (void)hipFree(tot_d);
+    if (tot_h) (void)hipHostFree(tot_h);
+    for (int i = 0; i < 2; i++) {
+        if (ev[i]) (void)hipEventDestroy(ev[i]);
+        if (kev[i]) (void)hipEventDestroy(kev[i]);
+    }
+    if (stream) (void)hipStreamDestroy(stream);
+    *this = Lane{};
+}
+
+Status ctx_open(int device, uint64_t budget, Ctx *&out)
+{
+    out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) return Status::err(GFSM_ERR_DEVICE, "no HIP device available");
+    if (device < 0 || device >= n) return Status::err(GFSM_ERR_ARG, "device index out of range");
+    HIP_TRY(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    std::unique_ptr<Ctx> c(new Ctx);
+    c->device = device;
+    c->budget = budget ? budget : (16ull << 30);
+    c->num_cus = (uint32_t)prop.multiProcessorCount;
+    c->max_tiles = 1ull << 22;
+    Status st = c->lane.init(c->max_tiles);
+    if (!st.good()) {
+        c->lane.destroy();
+        return st;
+    }
+    out = c.release();
+    return Status::ok();
+}
+
+static void free_mirror(DevGraph *dg)
+{
+    (void)hipSetDevice(dg->device);
+    (void)hipFree(dg->nbr);
+    (void)hipFree(dg->dir_ptr);
+    (void)hipFree(dg->dir);
+    (void)hipFree(dg->rel_vtx);
+    (void)hipFree(dg->rel_cnt);
+    delete dg;
+}
+
+void ctx_close(Ctx *ctx)
+{
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    {
+        std::lock_guard<std::mutex> lk(g_dev_mu);
+        for (const HostGraph *g : ctx->graphs) {
+            auto it = g->dev.find(ctx);
+            if (it != g->dev.end()) {
+                free_mirror(it->second);
+                g->dev.erase(it);
+            }
+        }
+        ctx->graphs.clear();
+    }
+    ctx->lane.destroy();
+    if (ctx->arena.base) (void)hipFree(ctx->arena.base);
+    delete ctx;
+}
+
+void release_graph_mirrors(const HostGraph &g)
+{
+    std::lock_guard<std::mutex> lk(g_dev_mu);
+    for (auto &kv : g.dev) {
+        Ctx *c = (Ctx *)const_cast<void *>(kv.first);
+        c->graphs.erase(&g);
+        free_mirror(kv.second);
+    }
+    g.dev.clear();
+}
+
+Status ensure_arena(Ctx &ctx, uint64_t bytes)
+{
+    if (ctx.arena.bytes >= bytes) return Status::ok();
+    if (ctx.arena.base) {
+        (void)hipFree(ctx.arena.base);
+        ctx.arena = Arena{};
+    }
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(&ctx.arena.base), bytes);
+    if (e != hipSuccess) {
+        ctx.arena = Arena{};
+        return Status::err(GFSM_ERR_NOMEM, "cannot allocate " + std::to_string(bytes) + " device bytes");
+    }
+    ctx.arena.bytes = bytes;
+    return Status::ok();
+}
+
+template <class T>
+static Status dev_copy(T *&dst, const std::vector<T> &src, size_t &bytes)
+{
+    size_t b = std::max<size_t>(src.size(), 1) * sizeof(T);
+    HIP_TRY(hipMalloc(&dst, b));
+    if (!src.empty()) HIP_TRY(hipMemcpy(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+    bytes += b;
+    return Status::ok();
+}
+
+Status upload_graph(Ctx &ctx, const HostGraph &g, DevGraph *&out)
+{
+    std::lock_guard<std::mutex> lk(g_dev_mu);
+    auto it = g.dev.find(&ctx);
+    if (it != g.dev.end()) {
+        out = it->second;
+        return Status::ok();
+    }
+    HIP_TRY(hipSetDevice(ctx.device));
+    DevGraph *dg = new DevGraph;
+    dg->device = ctx.device;
+    dg->ctx = &ctx;
+    std::vector<uint32_t> dir_flat(g.dir.size() * 2);
+    for (size_t i = 0; i < g.dir.size(); i++) {
+        dir_flat[2 * i] = g.dir[i].rel;
+        dir_flat[2 * i + 1] = g.dir[i].beg;
+    }
+    Status st = dev_copy(dg->nbr, g.nbr, dg->bytes);
+    if (st.good()) st = dev_copy(dg->dir_ptr, g.dir_ptr, dg->bytes);
+    if (st.good()) st = dev_copy(dg->dir, dir_flat, dg->bytes);
+    if (st.good()) st = dev_copy(dg->rel_vtx, g.rel_vtx, dg->bytes);
+    if (st.good()) st = dev_copy(dg->rel_cnt, g.rel_cnt, dg->bytes);
+    if (!st.good()) {
+        free_mirror(dg);
+        return st;
+    }
+    dg->view = DevGraphView{dg->nbr, dg->dir_ptr, dg->dir, dg->rel_vtx, dg->rel_cnt, g.V, (uint32_t)g.nent()};
+    g.dev[&ctx] = dg;
+    ctx.graphs.insert(&g);
+    out = dg;
+    return Status::ok();
+}
+
+void root_range(const HostGraph &data, const Plan &plan, uint32_t root_lo, uint32_t root_hi, uint64_t &lo,
+                uint64_t &hi)
+{
+    lo = hi = 0;
+    int64_t k = data.rel_index(plan.dev.root_rel);
+    if (k < 0) return;
+    lo = data.rel_off[k];
+    hi = data.rel_off[k + 1];
+    if (root_hi == 0 && root_lo == 0) return;
+    if (root_hi == 0) root_hi = 0xFFFFFFFFu;
+    const uint32_t *b = data.rel_vtx.data();
+    lo = std::lower_bound(b + lo, b + hi, root_lo) - b;
+    hi = std::lower_bound(b + lo, b + hi, root_hi) - b;
+}
+
+uint64_t alg_bytes(const Plan &plan, const gfsm_stats &st, uint32_t flags)
+{
+    const uint32_t q = plan.q;
+    uint64_t b = 4ull * st.roots;
+    for (uint32_t k = 1; k < q; k++) {
+        b += 8ull * st.level[k - 1];
+        b += 4ull * st.cand[k] * (1ull + plan.dev.depth[k].nchk);
+        if (k + 1 < q) b += 8ull * st.level[k];
+    }
+    if (flags & (GFSM_ROWS | GFSM_SORTED)) b += 4ull * q * st.level[q - 1];
+    return b;
+}
+
+Status match_single_vertex(const HostGraph &data, const Plan &plan, const gfsm_match_opts &o, Result &res)
+{
+    // A one-vertex query has no edges; the reference's root level lists every
+    // vertex that owns a CCSR row (HnGPU.cu:475-495), i.e. out-degree >= 1,
+    // and applies the vertex's self-loop requirements.
+    uint32_t lo = o.root_lo, hi = (o.root_hi || o.root_lo) ? o.root_hi : 0xFFFFFFFFu;
+    if (hi == 0) hi = 0xFFFFFFFFu;
+    res.count = 0;
+    for (uint32_t x = lo; x < data.V && x < hi; x++) {
+        if (data.degree(x) == 0) continue;
+        bool ok = true;
+        for (uint32_t r = 0; r < plan.dev.nroot_chk && ok; r++) {
+            uint32_t b, e;
+            data.segment(x, plan.dev.root_chk_rel[r], b, e);
+            ok = std::binary_search(data.nbr.begin() + b, data.nbr.begin() + e, x);
+        }
+        if (!ok) continue;
+        res.count++;
+        uint64_t h[2];
+        row_hash(&x, 1, h);
+        res.digest[0] += h[0];
+        res.digest[1] += h[1];
+        if (o.flags & (GFSM_ROWS | GFSM_SORTED)) res.rows.push_back(x);
+    }
+    res.stats.level[0] = res.count;
+    res.stats.roots = res.count;
+    if (o.max_rows && (o.flags & (GFSM_ROWS | GFSM_SORTED)) && res.count > o.max_rows)
+        return Status::err(GFSM_ERR_CAP, "more rows than max_rows");
+    return Status::ok();
+}
+
+namespace {
+
+// One plain match in flight: level buffers carved from the arena, the lane,
+// and the depth-first chunk recursion.
+struct Runner {
+    Ctx &ctx;
+    const HostGraph &data;
+    const DevGraph &dg;
+    const Plan &plan;
+    uint32_t flags;
+    uint64_t max_rows;
+    Result &res;
+    uint32_t q = 0;
+    bool rows_mode = false, digest = false, prof = false;
+    uint32_t *vtx[GFSM_MAXQ] = {}, *par[GFSM_MAXQ] = {};
+    uint64_t *off[GFSM_MAXQ] = {};
+    uint64_t cap[GFSM_MAXQ] = {};
+    uint64_t chunk[GFSM_MAXQ] = {};
+    uint32_t *rows_d = nullptr;
+    uint64_t rows_cap = 0;
+    uint32_t grid_cap = 0;
+
+    Runner(Ctx &c, const HostGraph &d, const DevGraph &g, const Plan &p, uint32_t f, uint64_t mr, Result &r)
+        : ctx(c), data(d), dg(g), plan(p), flags(f), max_rows(mr), res(r) {}
+
+    uint32_t grid_for(uint64_t ntiles) const
+    {
+        return (uint32_t)std::min<uint64_t>(ntiles, grid_cap);
+    }
+
+    Status carve(uint64_t budget)
+    {
+        q = plan.q;
+        rows_mode = flags & (GFSM_ROWS | GFSM_SORTED);
+        digest = (flags & GFSM_DIGEST) && !rows_mode;  // rows mode: digest from the host rows
+        prof = flags & GFSM_PROFILE;
+        grid_cap = std::max<uint32_t>(ctx.num_cus, 1) * 8;
+        uint64_t row_bytes = 0;
+        if (rows_mode) {
+            rows_cap = std::max<uint64_t>(budget / 4 / (4ull * q), kTile);
+            if (max_rows) rows_cap = std::min<uint64_t>(rows_cap, std::max<uint64_t>(max_rows, kTile));
+            row_bytes = (rows_cap * q * 4 + 255) & ~255ull;
+        }
+        if (budget <= row_bytes) return Status::err(GFSM_ERR_NOMEM, "memory budget too small");
+        const uint64_t per_level = (budget - row_bytes) / (q - 1);
+        const uint64_t c = per_level / 16 > 64 ? per_level / 16 - 64 : 0;
+        if (c < 4 * kTile) return Status::err(GFSM_ERR_NOMEM, "memory budget too small for the query depth");
+        uint64_t total = row_bytes;
+        for (uint32_t k = 0; k + 1 < q; k++) {
+            cap[k] = c;
+            total += (c * 4 + 255) / 256 * 256 * 2 + ((c + 1) * 8 + 255) / 256 * 256;
+            chunk[k + 1] = std::min<uint64_t>(c, ctx.max_tiles * kTile);
+        }
+        chunk[q - 1] = ctx.max_tiles * kTile;
+        Status st = ensure_arena(ctx, total);
+        if (!st.good()) return st;
+        char *p = ctx.arena.base;
+        auto take = [&p](uint64_t b) {
+            char *r = p;
+            p += (b + 255) / 256 * 256;
+            return r;
+        };
+        if (rows_mode) rows_d = reinterpret_cast<uint32_t *>(take(rows_cap * q * 4));
+        for (uint32_t k = 0; k + 1 < q; k++) {
+            vtx[k] = reinterpret_cast<uint32_t *>(take(cap[k] * 4));
+            par[k] = reinterpret_cast<uint32_t *>(take(cap[k] * 4));
+            off[k] = reinterpret_cast<uint64_t *>(take((cap[k] + 1) * 8));
+        }
+        return Status::ok();
+    }
+
+    Status sync_totals()
+    {
+        HIP_TRY(hipMemcpyAsync(ctx.lane.tot_h, ctx.lane.tot_d, sizeof(Totals), hipMemcpyDeviceToHost,
+                               ctx.lane.stream));
+        HIP_TRY(hipStreamSynchronize(ctx.lane.stream));
+        return Status::ok();
+    }
+
+    void kbegin()
+    {
+        if (prof) (void)hipEventRecord(ctx.lane.kev[0], ctx.lane.stream);
+    }
+    Status kend(bool expand)
+    {
+        if (!prof) return Status::ok();
+        HIP_TRY(hipEventRecord(ctx.lane.kev[1], ctx.lane.stream));
+        HIP_TRY(hipEventSynchronize(ctx.lane.kev[1]));
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, ctx.lane.kev[0], ctx.lane.kev[1]));
+        res.stats.kernel_ms += ms;
+        if (expand) {
+            res.stats.expand_ms += ms;
+            res.stats.expand_launches++;
+        }
+        return Status::ok();
+    }
+
+    Status run_roots(uint64_t rlo, uint64_t rhi)
+    {
+        res.stats.roots = rhi - rlo;
+        uint64_t step = std::min<uint64_t>(cap[0], ctx.max_tiles * kTile);
+        for (uint64_t c = rlo; c < rhi; c += step) {
+            uint64_t e = std::min(rhi, c + step);
+            RootArgs a{};
+            a.g = dg.view;
+            a.cand_vtx = dg.rel_vtx + c;
+            a.cand_cnt = dg.rel_cnt + c;
+            a.n = e - c;
+            a.nchk = plan.dev.nroot_chk;
+            std::memcpy(a.chk_rel, plan.dev.root_chk_rel, sizeof(a.chk_rel));
+            a.out_vtx = vtx[0];
+            a.out_par = par[0];
+            a.out_off = off[0];
+            a.cap = cap[0];
+            a.ws = ctx.lane.ws;
+            a.tot = ctx.lane.tot_d;
+            a.ntiles = (a.n + kTile - 1) / kTile;
+            if (a.ntiles > ctx.max_tiles || e > data.rel_vtx.size())
+                return Status::err(GFSM_ERR_ARG, "internal: root launch shape check failed");
+            kbegin();
+            HIP_TRY(launch_root(a, grid_for(a.ntiles), ctx.lane.stream));
+            res.stats.launches++;
+            Status st = kend(false);
+            if (!st.good()) return st;
+            st = sync_totals();
+            if (!st.good()) return st;
+            const uint64_t S0 = ctx.lane.tot_h->keep, G1 = ctx.lane.tot_h->cnt;
+            res.stats.level[0] += S0;
+            if (S0) {
+                st = descend(1, 0, G1, S0);
+                if (!st.good()) return st;
+            }
+        }
+        return Status::ok();
+    }
+
+    // Expand depth d over candidates [g0, g1) of level d-1 (S_prev nodes).
+    Status descend(uint32_t d, uint64_t g0, uint64_t g1, uint64_t S_prev)
+    {
+        const bool last = (d + 1 == q);
+        const uint32_t kind = last ? (rows_mode ? EXPAND_LAST_ROWS : EXPAND_LAST_COUNT) : EXPAND_MID;
+        const uint64_t tile_limit = ctx.max_tiles * kTile;
+        uint64_t first = true;
+        for (uint64_t c = g0; c < g1;) {
+            const uint64_t e = std::min(g1, c + std::max<uint64_t>(kTile, std::min(chunk[d], tile_limit)));
+            ExpandArgs a{};
+            a.g = dg.view;
+            a.dp = plan.dev.depth[d];
+            a.d = d;
+            a.q = q;
+            a.kind = kind;
+            a.digest = (last && digest) ? 1u : 0u;
+            std::memcpy(a.col, plan.dev.col, sizeof(a.col));
+            std::memcpy(a.dcol, plan.dev.dcol, sizeof(a.dcol));
+            a.off_prev = off[d - 1];
+            a.S_prev = S_prev;
+            a.g_begin = c;
+            a.g_end = e;
+            for (uint32_t k = 0; k < d; k++) {
+                a.lv.vtx[k] = vtx[k];
+                a.lv.par[k] = par[k];
+            }
+            if (!last) {
+                a.out_vtx = vtx[d];
+                a.out_par = par[d];
+                a.out_off = off[d];
+                a.cap = cap[d];
+            } else if (rows_mode) {
+                a.rows = rows_d;
+                a.cap = rows_cap;
+            }
+            a.ws = ctx.lane.ws;
+            a.tot = ctx.lane.tot_d;
+            a.ntiles = (e - c + kTile - 1) / kTile;
+            // Launch-shape checks: the workspace holds every tile, the frontier
+            // is non-empty and the LDS staging fits.
+            if (a.ntiles == 0 || a.ntiles > ctx.max_tiles || S_prev == 0 || S_prev > cap[d - 1] ||
+                a.dp.gen_dist == 0 || a.dp.gen_dist > d || expand_lds_bytes(d, a.dp.nchk) > 150 * 1024)
+                return Status::err(GFSM_ERR_ARG, "internal: expand launch shape check failed");
+            kbegin();
+            HIP_TRY(launch_expand(a, grid_for(a.ntiles), ctx.lane.stream));
+            res.stats.launches++;
+            if (!first) res.stats.chunks++;
+            first = false;
+            Status st = kend(true);
+            if (!st.good()) return st;
+            if (kind == EXPAND_LAST_COUNT) {
+                res.stats.cand[d] += e - c;
+                c = e;
+                continue;
+            }
+            st = sync_totals();
+            if (!st.good()) return st;
+            const uint64_t keep = ctx.lane.tot_h->keep;
+            const uint64_t limit = (kind == EXPAND_MID) ? cap[d] : rows_cap;
+            if (keep > limit) {
+                // The level did not fit: discard this launch and halve the range.
+                res.stats.overflows++;
+                chunk[d] = std::max<uint64_t>(kTile, (e - c) / 2);
+                continue;
+            }
+            res.stats.cand[d] += e - c;
+            res.stats.level[d] += keep;
+            if (keep) {
+                const double est = (double)limit * (double)(e - c) / (double)keep * 0.7;
+                chunk[d] = (uint64_t)std::max<double>(kTile, std::min<double>(est, (double)tile_limit));
+            } else {
+                chunk[d] = std::min<uint64_t>(chunk[d] * 4, tile_limit);
+            }
+            if (kind == EXPAND_MID) {
+                if (keep) {
+                    st = descend(d + 1, 0, ctx.lane.tot_h->cnt, keep);
+                    if (!st.good()) return st;
+                }
+            } else {
+                if (max_rows && res.rows.size() / q + keep > max_rows)
+                    return Status::err(GFSM_ERR_CAP, "more rows than max_rows");
+                size_t at = res.rows.size();
+                res.rows.resize(at + keep * q);
+                if (keep)
+                    HIP_TRY(hipMemcpy(res.rows.data() + at, rows_d, keep * q * sizeof(uint32_t),
+                                      hipMemcpyDeviceToHost));
+            }
+            c = e;
+        }
+        return Status::ok();
+    }
+};
+
+}  // namespace
+
+Status selftest_scan(Ctx &ctx, const uint32_t *vals, uint64_t n, uint64_t *prefix, uint32_t *idx, uint64_t totals[2])
+{
+    std::lock_guard<std::mutex> lk(ctx.mu);
+    HIP_TRY(hipSetDevice(ctx.device));
+    totals[0] = totals[1] = 0;
+    if (n == 0) return Status::ok();
+    const uint64_t ntiles = (n + kTile - 1) / kTile;
+    if (ntiles > ctx.max_tiles) return Status::err(GFSM_ERR_ARG, "too many values for one launch");
+    uint32_t *dv = nullptr, *di = nullptr;
+    uint64_t *dp = nullptr;
+    Status st = Status::ok();
+    auto cleanup = [&] {
+        (void)hipFree(dv);
+        (void)hipFree(di);
+        (void)hipFree(dp);
+    };
+    hipError_t e = hipMalloc(&dv, n * 4);
+    if (e == hipSuccess) e = hipMalloc(&di, n * 4);
+    if (e == hipSuccess) e = hipMalloc(&dp, n * 8);
+    if (e == hipSuccess) e = hipMemcpy(dv, vals, n * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(di, 0xFF, n * 4);
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(ntiles, (uint64_t)std::max<uint32_t>(ctx.num_cus, 1) * 8);
+    if (e == hipSuccess) e = launch_scan_test(dv, n, dp, di, ctx.lane.ws, ctx.lane.tot_d, grid, ctx.lane.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx.lane.stream);
+    if (e == hipSuccess) e = hipMemcpy(prefix, dp, n * 8, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(idx, di, n * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(ctx.lane.tot_h, ctx.lane.tot_d, sizeof(Totals), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) st = hip_status(e, "selftest_scan");
+    else {
+        totals[0] = ctx.lane.tot_h->keep;
+        totals[1] = ctx.lane.tot_h->cnt;
+    }
+    cleanup();
+    return st;
+}
+
+Status match_plain(Ctx &ctx, const HostGraph &data, const HostGraph &query, const gfsm_match_opts &o, Result &res)
+{
+    const double t0 = now_ms();
+    res = Result{};
+    if (data.mode != query.mode) return Status::err(GFSM_ERR_ARG, "query and data graph use different modes");
+    Plan plan;
+    Status st = plan_query(query, plan);
+    if (!st.good()) return st;
+    res.width = plan.q;
+    res.stats.q = plan.q;
+    if (plan.q == 1) {
+        st = match_single_vertex(data, plan, o, res);
+        res.stats.host_ms = now_ms() - t0;
+        return st;
+    }
+    std::lock_guard<std::mutex> lk(ctx.mu);
+    HIP_TRY(hipSetDevice(ctx.device));
+    DevGraph *dg = nullptr;
+    st = upload_graph(ctx, data, dg);
+    if (!st.good()) return st;
+
+    Runner run(ctx, data, *dg, plan, o.flags, o.max_rows, res);
+    st = run.carve(o.mem_budget ? o.mem_budget : ctx.budget);
+    if (!st.good()) return st;
+    uint64_t rlo, rhi;
+    root_range(data, plan, o.root_lo, o.root_hi, rlo, rhi);
+
+    Lane &L = ctx.lane;
+    HIP_TRY(hipMemsetAsync(L.tot_d, 0, sizeof(Totals), L.stream));
+    HIP_TRY(hipEventRecord(L.ev[0], L.stream));
+    st = run.run_roots(rlo, rhi);
+    if (!st.good()) {
+        (void)hipStreamSynchronize(L.stream);
+        return st;
+    }
+    HIP_TRY(hipEventRecord(L.ev[1], L.stream));
+    st = run.sync_totals();
+    if (!st.good()) return st;
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, L.ev[0], L.ev[1]));
+    res.stats.device_ms = ms;
+    const uint32_t q = plan.q;
+    if (run.rows_mode) {
+        res.count = res.rows.size() / q;
+        res.stats.level[q - 1] = res.count;
+        if (o.flags & GFSM_DIGEST) {
+            uint64_t d0 = 0, d1 = 0;
+            for (uint64_t r = 0; r < res.count; r++) {
+                uint64_t h[2];
+                row_hash(res.rows.data() + r * q, q, h);
+                d0 += h[0];
+                d1 += h[1];
+            }
+            res.digest[0] = d0;
+            res.digest[1] = d1;
+        }
+        if (o.flags & GFSM_SORTED) sort_rows(res.rows, q);
+    } else {
+        res.count = L.tot_h->count;
+        res.stats.level[q - 1] = res.count;
+        res.digest[0] = L.tot_h->dg0;
+        res.digest[1] = L.tot_h->dg1;
+    }
+    for (uint32_t k = 1; k < q; k++) res.stats.checks[k] = plan.dev.depth[k].nchk;
+    res.stats.alg_bytes = alg_bytes(plan, res.stats, o.flags);
+    res.stats.host_ms = now_ms() - t0;
+    return Status::ok();
+}
+
+}  // namespace gfsm

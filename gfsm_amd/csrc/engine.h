@@ -1,0 +1,83 @@
+// engine.h -- device contexts, resident graphs and the plain matcher driver.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <atomic>
+#include <memory>
+#include <mutex>
+#include <set>
+
+#include "host.h"
+#include "kernels.h"
+
+namespace gfsm {
+
+struct DevGraph {
+    int device = 0;
+    const void *ctx = nullptr;
+    uint32_t *nbr = nullptr, *dir_ptr = nullptr, *dir = nullptr, *rel_vtx = nullptr, *rel_cnt = nullptr;
+    DevGraphView view{};
+    size_t bytes = 0;
+};
+
+// A stream with its own chained-scan workspace and result block.  Kernels of
+// one lane run in stream order, so the workspace is never shared by two
+// launches at once.
+struct Lane {
+    hipStream_t stream = nullptr;
+    ScanWS ws{};
+    Totals *tot_d = nullptr;
+    Totals *tot_h = nullptr;  // pinned
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipEvent_t kev[2] = {nullptr, nullptr};
+    Status init(uint64_t max_tiles);
+    void destroy();
+};
+
+// Device arena for intermediate levels and output rows.
+struct Arena {
+    char *base = nullptr;
+    uint64_t bytes = 0;
+};
+
+struct Ctx {
+    int device = 0;
+    uint64_t budget = 0;
+    uint32_t num_cus = 0;
+    uint64_t max_tiles = 0;
+    Lane lane;
+    Arena arena;
+    std::mutex mu;
+    std::set<const HostGraph *> graphs;  // data graphs resident on this context
+};
+
+Status ctx_open(int device, uint64_t budget, Ctx *&out);
+void ctx_close(Ctx *ctx);
+Status ensure_arena(Ctx &ctx, uint64_t bytes);
+Status upload_graph(Ctx &ctx, const HostGraph &g, DevGraph *&out);
+void release_graph_mirrors(const HostGraph &g);
+
+// Plain single-query path: host-driven depth loop with chunked expansion.
+Status match_plain(Ctx &ctx, const HostGraph &data, const HostGraph &query, const gfsm_match_opts &opts,
+                   Result &res);
+
+// Matcher helpers shared with the batched and serving paths.
+Status match_single_vertex(const HostGraph &data, const Plan &plan, const gfsm_match_opts &opts, Result &res);
+void root_range(const HostGraph &data, const Plan &plan, uint32_t root_lo, uint32_t root_hi, uint64_t &lo,
+                uint64_t &hi);
+Status selftest_scan(Ctx &ctx, const uint32_t *vals, uint64_t n, uint64_t *prefix, uint32_t *idx,
+                     uint64_t totals[2]);
+uint64_t alg_bytes(const Plan &plan, const gfsm_stats &st, uint32_t flags);
+
+inline Status hip_status(hipError_t e, const char *what)
+{
+    if (e == hipSuccess) return Status::ok();
+    return Status::err(GFSM_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+}  // namespace gfsm
+
+struct gfsm_ctx {
+    gfsm::Ctx *c;
+};

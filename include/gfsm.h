@@ -1,0 +1,203 @@
+/*
+ * gfsm.h -- C ABI of libgfsm.so, the MI355X-native forest-based subgraph
+ * matcher.  Plain pointers and sizes only; every call returns a gfsm_status
+ * and never exits the process.
+ *
+ * What each entry point replaces in the reference (paths relative to the
+ * reference repository root).  The reference has no C/FFI surface: its
+ * boundary is the process `GFSM <query> <data>` (main.cu:56-63) plus the C++
+ * namespace HnSetup (HnGPU.cuh:60-80), so each entry point below names the
+ * reference code whose job it takes over:
+ *
+ *   gfsm_open / gfsm_close      HnSetup::preinit (HnGPU.cu:847-862) + the device
+ *                               reset/warm-up of argTest (main.cu:11-20)
+ *   gfsm_graph_load_tve         fileParse (HnHelper.cuh:283-499)
+ *   gfsm_graph_from_edges       fileParse's row staging + CCSR layout
+ *                               (HnHelper.cuh:364-478) from in-memory arrays
+ *   gfsm_plan_query             HnSetup::preProcessQuery (HnGPU.cu:749-816) +
+ *                               CCSR::genereateCCSRRequirements (ccsr.cpp:98-204)
+ *   gfsm_match                  HnSetup::solve -> gpuLaunch -> hnSolve
+ *                               (HnGPU.cu:710-739, 867-1063, 455-669), with the
+ *                               overflow retry (HnGPU.cu:957-992) replaced by
+ *                               chunked expansion
+ *   gfsm_result_count           the "Function Count" CSV field (HnGPU.cu:999-1000)
+ *   gfsm_result_rows            hnWrite + D2H of the table (HnGPU.cu:674-699, 1019-1048)
+ *   gfsm_result_write_fs        HnSetup::functionsToFile (HnGPU.cu:1081-1091)
+ *   gfsm_result_digest          (no counterpart) order-independent row digest
+ *   gfsm_match_batch            (no counterpart) many queries, one persistent launch per depth
+ *   gfsm_server_*               (no counterpart) resident-graph serving loop
+ *   gfsm_gen_*                  (no counterpart) seeded workload generators
+ *
+ * Matching semantics (identical to the reference on valid inputs): an
+ * embedding is an injective map f from query vertices to data vertices such
+ * that, when query vertex n is placed, every query entry n->u to an already
+ * placed u has a data entry f(u)->f(n) with the relation of u->n (or of n->u
+ * when the query has no u->n entry).  Non-induced; automorphic images count
+ * separately.  GFSM_MODE_REF: relation = edge label, vertex labels ignored.
+ * GFSM_MODE_VLABEL: relation of u->v = (label(u), label(v), edge label).
+ *
+ * Threading: a context is used by one host thread at a time; a server is
+ * safe for concurrent submit/poll.  Ownership: every handle is released by
+ * its own *_free / *_close / *_stop.
+ */
+#ifndef GFSM_H
+#define GFSM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t gfsm_status;
+/* Status codes; 1 and 232-234 are the reference's own exit codes. */
+#define GFSM_OK 0
+#define GFSM_ERR_QUERY 1          /* disconnected or invalid query (ccsr.cpp:126-128, 192-196) */
+#define GFSM_ERR_ARG 2            /* bad argument / out-of-domain input                      */
+#define GFSM_ERR_NOMEM 3          /* device or host allocation failed, or budget too small   */
+#define GFSM_ERR_DEVICE 4         /* HIP runtime error                                       */
+#define GFSM_ERR_CAP 5            /* more rows than the caller's max_rows                    */
+#define GFSM_ERR_PENDING 6        /* server: result not ready yet                            */
+#define GFSM_ERR_MISSING_FILE 232 /* HnHelper.cuh:16 */
+#define GFSM_ERR_INVALID_DATA 233 /* HnHelper.cuh:17 */
+#define GFSM_ERR_MISSING_DATA 234 /* HnHelper.cuh:18 */
+
+#define GFSM_MODE_REF 0u
+#define GFSM_MODE_VLABEL 1u
+
+/* gfsm_match_opts.flags */
+#define GFSM_ROWS 1u     /* materialise the embedding table on the host           */
+#define GFSM_SORTED 2u   /* ... sorted lexicographically by (f(0), ..., f(q-1))    */
+#define GFSM_DIGEST 4u   /* compute the order-independent multiset digest         */
+#define GFSM_PROFILE 8u  /* time every kernel launch with HIP events (stats)      */
+
+#define GFSM_MAX_Q 32u   /* largest query handled (vertices)                      */
+
+typedef struct gfsm_ctx gfsm_ctx;
+typedef struct gfsm_graph gfsm_graph;
+typedef struct gfsm_result gfsm_result;
+typedef struct gfsm_server gfsm_server;
+
+typedef struct gfsm_match_opts {
+    uint32_t flags;
+    uint32_t root_lo, root_hi;  /* only embeddings with f(root) in [root_lo, root_hi); 0,0 = all */
+    uint32_t tile;              /* candidates per workgroup tile (0 = default)                    */
+    uint64_t max_rows;          /* cap on materialised rows (0 = no cap)                          */
+    uint64_t mem_budget;        /* device bytes for intermediate levels (0 = context default)     */
+} gfsm_match_opts;
+
+typedef struct gfsm_stats {
+    uint32_t q;                 /* query vertices                                            */
+    uint32_t launches;          /* matcher kernel launches                                   */
+    uint32_t chunks;            /* expand launches beyond one per depth (chunked expansion)  */
+    uint32_t overflows;         /* launches discarded because a level overflowed its buffer  */
+    uint64_t roots;             /* root candidates read                                      */
+    uint64_t level[GFSM_MAX_Q]; /* nodes kept per depth (last depth: embeddings)             */
+    uint64_t cand[GFSM_MAX_Q];  /* candidates generated per depth                            */
+    uint32_t checks[GFSM_MAX_Q];/* membership probes per candidate at each depth             */
+    uint64_t alg_bytes;         /* algorithmic HBM bytes of the path (see design notes)      */
+    double device_ms;           /* first matcher kernel -> result ready (HIP events)         */
+    double kernel_ms;           /* sum of matcher kernel durations (GFSM_PROFILE only)       */
+    double expand_ms;           /* sum of expand-kernel durations (GFSM_PROFILE only)        */
+    uint32_t expand_launches;   /* expand-kernel launches timed in expand_ms                 */
+    uint32_t pad;
+    double host_ms;             /* wall time of the call                                     */
+} gfsm_stats;
+
+/* ---- context ------------------------------------------------------------ */
+gfsm_status gfsm_open(int device, uint64_t mem_budget_bytes, gfsm_ctx **ctx);
+void        gfsm_close(gfsm_ctx *ctx);
+/* Last error message of the calling thread ("" when none). */
+const char *gfsm_last_error(void);
+/* Library version string. */
+const char *gfsm_version(void);
+
+/* ---- graphs (host objects; uploaded to a context on first use, then resident) */
+gfsm_status gfsm_graph_load_tve(const char *path, uint32_t mode, gfsm_graph **g);
+gfsm_status gfsm_graph_from_edges(uint32_t nv, const uint32_t *vlabel, uint64_t ne,
+                                  const uint32_t *src, const uint32_t *dst, const uint32_t *elabel,
+                                  uint32_t mode, gfsm_graph **g);
+gfsm_status gfsm_graph_write_tve(const gfsm_graph *g, const char *path);
+/* Views of the graph's input arrays (valid while g lives). */
+gfsm_status gfsm_graph_edges(const gfsm_graph *g, uint32_t *nv, uint64_t *ne, const uint32_t **vlabel,
+                             const uint32_t **src, const uint32_t **dst, const uint32_t **elabel);
+/* Views of the relation-partitioned layout: dir_ptr[nv+1]; dir[2*(ndir+1)] as
+ * {relation, begin} pairs; nbr[nent]. */
+gfsm_status gfsm_graph_layout(const gfsm_graph *g, uint32_t *nv, uint64_t *ndir, uint64_t *nent,
+                              const uint32_t **dir_ptr, const uint32_t **dir, const uint32_t **nbr);
+gfsm_status gfsm_graph_info(const gfsm_graph *g, uint32_t *nv, uint64_t *nent, uint32_t *max_degree,
+                            uint32_t *mode);
+/* Pre-upload a data graph to a context (optional; gfsm_match does it lazily). */
+gfsm_status gfsm_graph_upload(gfsm_ctx *ctx, const gfsm_graph *g);
+void        gfsm_graph_free(gfsm_graph *g);
+
+/* ---- query planning (host only) -------------------------------------------
+ * Writes the reference's requirement layout for the query: col[q] = original
+ * query id placed at depth d; header[3q] = {num, offset, firstQ};
+ * requirements[2*maxval*q] = {delta distance, relation}.  Pass NULL arrays to
+ * learn q and maxval first. */
+gfsm_status gfsm_plan_query(const gfsm_graph *query, uint32_t *q, uint32_t *maxval, uint32_t *col,
+                            uint32_t *header, uint32_t *requirements);
+
+/* ---- matching ------------------------------------------------------------ */
+gfsm_status gfsm_match(gfsm_ctx *ctx, const gfsm_graph *data, const gfsm_graph *query,
+                       const gfsm_match_opts *opts, gfsm_result **r);
+gfsm_status gfsm_result_count(const gfsm_result *r, uint64_t *n);
+gfsm_status gfsm_result_rows(const gfsm_result *r, const uint32_t **rows, uint32_t *width, uint64_t *nrows);
+gfsm_status gfsm_result_digest(const gfsm_result *r, uint64_t out[2]);
+gfsm_status gfsm_result_stats(const gfsm_result *r, gfsm_stats *out);
+gfsm_status gfsm_result_status(const gfsm_result *r);
+gfsm_status gfsm_result_write_fs(const gfsm_result *r, const char *path);
+void        gfsm_result_free(gfsm_result *r);
+
+/* Row digest used by GFSM_DIGEST: h = sum over rows of two seeded hashes of
+ * the row (column order = original query id). */
+void        gfsm_row_hash(const uint32_t *row, uint32_t width, uint64_t h[2]);
+
+/* ---- batching: n queries share each depth step; one persistent launch per
+ * depth with a device work queue of (query, tile) items ------------------ */
+gfsm_status gfsm_match_batch(gfsm_ctx *ctx, const gfsm_graph *data, const gfsm_graph *const *queries,
+                             uint32_t n, const gfsm_match_opts *opts, gfsm_result **results);
+
+/* ---- serving: resident data graph, admission by projected memory,
+ * continuous batching -------------------------------------------------------- */
+typedef struct gfsm_serve_opts {
+    uint64_t mem_budget;    /* device bytes shared by in-flight queries (0 = context default) */
+    uint32_t max_inflight;  /* queries admitted at once (0 = 256)                              */
+    uint32_t flags;         /* GFSM_DIGEST etc., applied to every query                        */
+} gfsm_serve_opts;
+
+gfsm_status gfsm_server_start(gfsm_ctx *ctx, const gfsm_graph *data, const gfsm_serve_opts *opts,
+                              gfsm_server **srv);
+gfsm_status gfsm_server_submit(gfsm_server *srv, const gfsm_graph *query, uint64_t *ticket);
+/* GFSM_ERR_PENDING while running; on GFSM_OK *r is owned by the caller. */
+gfsm_status gfsm_server_poll(gfsm_server *srv, uint64_t ticket, gfsm_result **r);
+gfsm_status gfsm_server_wait(gfsm_server *srv, uint64_t ticket, gfsm_result **r);
+/* Per-ticket timestamps (seconds on the server clock): submit, admit, done. */
+gfsm_status gfsm_server_times(gfsm_server *srv, uint64_t ticket, double t[3]);
+void        gfsm_server_stop(gfsm_server *srv);
+
+/* ---- self test ------------------------------------------------------------
+ * (no counterpart) Runs the chained wave64 scan on n u32 values: prefix[i] =
+ * sum of vals[0..i) (u64), idx = the indices with vals[i] != 0 in order, and
+ * totals[0] = number of non-zero values, totals[1] = sum.  Host arrays. */
+gfsm_status gfsm_selftest_scan(gfsm_ctx *ctx, const uint32_t *vals, uint64_t n, uint64_t *prefix,
+                               uint32_t *idx, uint64_t totals[2]);
+
+/* ---- seeded workload generators (undirected graphs as symmetric entry lists) */
+#define GFSM_WALK_DENSE 0u   /* keep every induced edge among the walked vertices */
+#define GFSM_WALK_SPARSE 1u  /* walk tree plus one extra induced edge             */
+gfsm_status gfsm_gen_er(uint32_t n, double avg_degree, uint32_t nvlabels, uint32_t nelabels,
+                        uint64_t seed, uint32_t mode, gfsm_graph **g);
+gfsm_status gfsm_gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, double c,
+                          uint32_t nvlabels, double zipf_s, uint32_t nelabels, uint64_t seed,
+                          uint32_t mode, gfsm_graph **g);
+gfsm_status gfsm_gen_grid(uint32_t rows, uint32_t cols, double del_frac, double diag_frac,
+                          uint64_t seed, uint32_t mode, gfsm_graph **g);
+gfsm_status gfsm_gen_query_walk(const gfsm_graph *data, uint32_t qsize, uint32_t kind, uint64_t seed,
+                                gfsm_graph **q);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

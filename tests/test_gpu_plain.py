@@ -1,0 +1,122 @@
+"""GPU parity of the plain (single-query) path against the CPU oracle."""
+import numpy as np
+import pytest
+
+import gfsm_amd as G
+import oracle as O
+from helpers import assert_same, oracle_solve, to_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 255, 256, 257, 1000, 65537, 1 << 20])
+def test_chained_scan_random(ctx, n):
+    rng = np.random.default_rng(n)
+    vals = rng.integers(0, 5, size=n).astype(np.uint32)
+    vals[rng.random(n) < 0.3] = 0
+    pre, idx, (nk, tot) = ctx.selftest_scan(vals)
+    ref = np.concatenate([[0], np.cumsum(vals.astype(np.uint64))[:-1]]).astype(np.uint64)
+    assert np.array_equal(pre, ref)
+    assert np.array_equal(idx, np.nonzero(vals)[0].astype(np.uint32))
+    assert nk == int((vals != 0).sum()) and tot == int(vals.astype(np.uint64).sum())
+
+
+@pytest.mark.parametrize("kind", ["zeros", "ones", "big"])
+def test_chained_scan_adversarial(ctx, kind):
+    n = 300_000
+    if kind == "zeros":
+        vals = np.zeros(n, np.uint32)
+    elif kind == "ones":
+        vals = np.ones(n, np.uint32)
+    else:  # per-wave sums stay below 2^32, the total exceeds it
+        vals = np.full(n, (1 << 25) + 12345, np.uint32)
+    pre, idx, (nk, tot) = ctx.selftest_scan(vals)
+    ref = np.concatenate([[0], np.cumsum(vals.astype(np.uint64))[:-1]]).astype(np.uint64)
+    assert np.array_equal(pre, ref)
+    assert nk == int((vals != 0).sum()) and tot == int(vals.astype(np.uint64).sum())
+    assert tot < (1 << 32) or kind == "big"
+
+
+def test_readme_graph(ctx, tmp_path):
+    p = tmp_path / "readme.g"
+    p.write_text("t # 0\n2 2 1 1\nv 0 1\nv 1 1\ne 0 1 1\ne 1 0 1\nt # -1")
+    g = G.Graph.load(p)
+    m = ctx.match(g, g, rows=True, sorted=True)
+    assert m.count == 2
+    assert m.rows.tolist() == [[0, 1], [1, 0]]
+
+
+@pytest.mark.parametrize("mode", [G.MODE_REF, G.MODE_VLABEL])
+@pytest.mark.parametrize("qsize,kind,seed", [(3, G.WALK_DENSE, 1), (4, G.WALK_DENSE, 2), (5, G.WALK_DENSE, 3),
+                                             (5, G.WALK_SPARSE, 4), (6, G.WALK_SPARSE, 5), (7, G.WALK_DENSE, 6),
+                                             (2, G.WALK_DENSE, 7)])
+def test_er_vs_oracle(ctx, mode, qsize, kind, seed):
+    data = G.Graph.er(600, 6.0, nvlabels=3, nelabels=2, seed=100 + seed, mode=mode)
+    query = G.Graph.walk_query(data, qsize, kind=kind, seed=seed)
+    m = ctx.match(data, query, rows=True, sorted=True, digest=True)
+    ref = oracle_solve(data, query, mode)
+    assert_same(m, ref)
+
+
+def test_config1_sorted_rows(ctx):
+    data = G.Graph.er(2000, 8.0, nvlabels=4, nelabels=1, seed=1000, mode=G.MODE_VLABEL)
+    query = G.Graph.walk_query(data, 5, kind=G.WALK_DENSE, seed=1001)
+    m = ctx.match(data, query, rows=True, sorted=True, digest=True)
+    ref = oracle_solve(data, query, G.MODE_VLABEL)
+    assert_same(m, ref)
+    # soundness restated from the reference's checker
+    assert O.check_sound(to_oracle(query), to_oracle(data), m.rows) == m.count
+
+
+@pytest.mark.parametrize("seed", [11, 12, 13])
+def test_rmat_count_digest(ctx, seed):
+    data = G.Graph.rmat(12, 8, nvlabels=4, zipf_s=1.0, nelabels=2, seed=seed, mode=G.MODE_VLABEL)
+    for qs, kind in [(4, G.WALK_DENSE), (5, G.WALK_SPARSE), (6, G.WALK_DENSE)]:
+        query = G.Graph.walk_query(data, qs, kind=kind, seed=seed * 10 + qs)
+        m = ctx.match(data, query, digest=True)
+        ref = oracle_solve(data, query, G.MODE_VLABEL, rows=False)
+        assert m.count == ref.count and m.digest == ref.digest, (qs, kind, m.count, ref.count)
+
+
+def test_grid_closed_forms(ctx):
+    R, Cc = 30, 40
+    g = G.Graph.grid(R, Cc)
+    import os
+    qdir = os.path.join(os.path.dirname(__file__), "golden", "queries")
+    def q(name):
+        return G.Graph.load(os.path.join(qdir, name))
+    deg = np.array([(r > 0) + (r < R - 1) + (c > 0) + (c < Cc - 1) for r in range(R) for c in range(Cc)])
+    expect = {
+        "road2.g": 2 * (R * (Cc - 1) + Cc * (R - 1)),
+        "road3.g": int((deg * (deg - 1)).sum()),
+        "triangle.g": 0,
+        "square.g": 8 * (R - 1) * (Cc - 1),
+        "roadY.g": int((deg * (deg - 1) * (deg - 2)).sum()),
+        "plus.g": 24 * (R - 2) * (Cc - 2),
+    }
+    for name, want in expect.items():
+        m = ctx.match(g, q(name), digest=True)
+        assert m.count == want, (name, m.count, want)
+
+
+def test_chunked_equals_unchunked(ctx):
+    data = G.Graph.grid(60, 60, del_frac=0.05, diag_frac=0.02, seed=3)
+    query = G.Graph.walk_query(data, 6, kind=G.WALK_SPARSE, seed=9)
+    full = ctx.match(data, query, digest=True)
+    small = ctx.match(data, query, digest=True, mem_budget=1 << 20)
+    assert full.count == small.count and full.digest == small.digest
+    assert small.stats["chunks"] > 0
+    rows_full = ctx.match(data, query, rows=True, sorted=True)
+    rows_small = ctx.match(data, query, rows=True, sorted=True, mem_budget=1 << 20)
+    assert np.array_equal(rows_full.rows, rows_small.rows)
+
+
+def test_root_range_partition(ctx):
+    data = G.Graph.er(1500, 7.0, nvlabels=2, seed=77, mode=G.MODE_VLABEL)
+    query = G.Graph.walk_query(data, 5, kind=G.WALK_SPARSE, seed=78)
+    full = ctx.match(data, query, digest=True)
+    parts = [ctx.match(data, query, digest=True, root_lo=lo, root_hi=lo + 500) for lo in (0, 500, 1000)]
+    assert sum(p.count for p in parts) == full.count
+    d0 = sum(p.digest[0] for p in parts) % (1 << 64)
+    d1 = sum(p.digest[1] for p in parts) % (1 << 64)
+    assert (d0, d1) == full.digest

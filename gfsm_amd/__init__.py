@@ -44,7 +44,8 @@ class GfsmError(RuntimeError):
 
 class MatchOpts(C.Structure):
     _fields_ = [("flags", C.c_uint32), ("root_lo", C.c_uint32), ("root_hi", C.c_uint32),
-                ("tile", C.c_uint32), ("max_rows", C.c_uint64), ("mem_budget", C.c_uint64)]
+                ("tile", C.c_uint32), ("max_rows", C.c_uint64), ("mem_budget", C.c_uint64),
+                ("max_cand", C.c_uint64)]
 
 
 class Stats(C.Structure):
@@ -288,10 +289,10 @@ class Match:
 
 
 def _opts(rows=False, sorted=False, digest=False, profile=False, root_lo=0, root_hi=0, max_rows=0,
-          mem_budget=0):
+          mem_budget=0, max_cand=0):
     flags = (ROWS if rows else 0) | (SORTED if sorted else 0) | (DIGEST if digest else 0) | \
         (PROFILE if profile else 0)
-    return MatchOpts(flags, root_lo, root_hi, 0, max_rows, mem_budget)
+    return MatchOpts(flags, root_lo, root_hi, 0, max_rows, mem_budget, max_cand)
 
 
 class Context:
@@ -329,10 +330,18 @@ class Context:
     def upload(self, g: Graph):
         _check(lib().gfsm_graph_upload(self._h, g._h), "upload")
 
-    def match(self, data: Graph, query: Graph, **kw) -> Match:
+    def match(self, data: Graph, query: Graph, allow_partial=False, **kw) -> Match:
+        """Match query in data.  With a work cap (max_cand) and allow_partial,
+        a capped run returns its partial Match (status == 5) instead of raising."""
         o = _opts(**kw)
         r = C.c_void_p()
-        _check(lib().gfsm_match(self._h, data._h, query._h, C.byref(o), C.byref(r)), "match")
+        rc = lib().gfsm_match(self._h, data._h, query._h, C.byref(o), C.byref(r))
+        if rc == 5 and r.value:
+            m = Match(r)
+            if allow_partial:
+                return m
+            del m
+        _check(rc, "match")
         return Match(r)
 
     def match_batch(self, data: Graph, queries, **kw):

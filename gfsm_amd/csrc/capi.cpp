@@ -179,6 +179,10 @@ gfsm_status gfsm_match(gfsm_ctx *ctx, const gfsm_graph *data, const gfsm_graph *
     if (opts) o = *opts;
     std::unique_ptr<gfsm_result> res(new gfsm_result);
     Status st = match_plain(*ctx->c, *data->g, *query->g, o, res->r);
+    if (st.code == GFSM_ERR_CAP && res->r.status == GFSM_ERR_CAP) {
+        *r = res.release();  // partial result under the work cap
+        return fail(st);
+    }
     if (!st.good()) return fail(st);
     *r = res.release();
     return ok();

@@ -39,11 +39,11 @@ Status Lane::init(uint64_t max_tiles)
 {
     HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     ws.max_tiles = max_tiles;
-    HIP_TRY(hipMalloc(&ws.flag, max_tiles * sizeof(uint32_t)));
-    HIP_TRY(hipMalloc(&ws.agg, max_tiles * sizeof(Pair64)));
-    HIP_TRY(hipMalloc(&ws.inc, max_tiles * sizeof(Pair64)));
+    HIP_TRY(hipMalloc(&ws.wk, max_tiles * sizeof(uint64_t)));
+    HIP_TRY(hipMalloc(&ws.wc, max_tiles * sizeof(uint64_t)));
     HIP_TRY(hipMalloc(&ws.ctl, 4 * sizeof(uint32_t)));
-    HIP_TRY(hipMemset(ws.flag, 0, max_tiles * sizeof(uint32_t)));
+    HIP_TRY(hipMemset(ws.wk, 0, max_tiles * sizeof(uint64_t)));
+    HIP_TRY(hipMemset(ws.wc, 0, max_tiles * sizeof(uint64_t)));
     HIP_TRY(hipMemset(ws.ctl, 0, 4 * sizeof(uint32_t)));
     HIP_TRY(hipMalloc(&tot_d, sizeof(Totals)));
     HIP_TRY(hipMemset(tot_d, 0, sizeof(Totals)));
@@ -56,12 +56,21 @@ Status Lane::init(uint64_t max_tiles)
     return Status::ok();
 }
 
+hipError_t Lane::before_launch()
+{
+    // Launch L runs with epoch L mod 2^16 (the device counter advances once per
+    // launch); a word left by launch L - 2^16 would carry the same epoch.
+    hipError_t e = hipSuccess;
+    if (launches && (launches % (1ull << kEpochBits)) == 0) e = launch_ws_clear(ws, stream);
+    launches++;
+    return e;
+}
+
 void Lane::destroy()
 {
     if (stream) (void)hipStreamSynchronize(stream);
-    (void)hipFree(ws.flag);
-    (void)hipFree(ws.agg);
-    (void)hipFree(ws.inc);
+    (void)hipFree(ws.wk);
+    (void)hipFree(ws.wc);
     (void)hipFree(ws.ctl);
     (void)hipFree(tot_d);
     if (tot_h) (void)hipHostFree(tot_h);
@@ -278,9 +287,21 @@ struct Runner {
     uint32_t *rows_d = nullptr;
     uint64_t rows_cap = 0;
     uint32_t grid_cap = 0;
+    uint64_t max_cand = 0;    // work cap (0 = none)
+    uint64_t cand_total = 0;  // candidates launched so far
 
     Runner(Ctx &c, const HostGraph &d, const DevGraph &g, const Plan &p, uint32_t f, uint64_t mr, Result &r)
         : ctx(c), data(d), dg(g), plan(p), flags(f), max_rows(mr), res(r) {}
+
+    // Candidates per launch: the workspace holds max_tiles tiles, and the
+    // launch's next-depth candidate total (at most candidates x max degree)
+    // must fit the 46-bit value field of the scan words.
+    uint64_t launch_limit() const
+    {
+        const uint64_t by_ws = ctx.max_tiles * kTileCand;
+        const uint64_t by_val = kValueMask / std::max<uint64_t>(1, data.max_degree);
+        return std::max<uint64_t>(kTileCand, std::min(by_ws, by_val));
+    }
 
     uint32_t grid_for(uint64_t ntiles) const
     {
@@ -296,21 +317,21 @@ struct Runner {
         grid_cap = std::max<uint32_t>(ctx.num_cus, 1) * 8;
         uint64_t row_bytes = 0;
         if (rows_mode) {
-            rows_cap = std::max<uint64_t>(budget / 4 / (4ull * q), kTile);
-            if (max_rows) rows_cap = std::min<uint64_t>(rows_cap, std::max<uint64_t>(max_rows, kTile));
+            rows_cap = std::max<uint64_t>(budget / 4 / (4ull * q), kTileCand);
+            if (max_rows) rows_cap = std::min<uint64_t>(rows_cap, std::max<uint64_t>(max_rows, kTileCand));
             row_bytes = (rows_cap * q * 4 + 255) & ~255ull;
         }
         if (budget <= row_bytes) return Status::err(GFSM_ERR_NOMEM, "memory budget too small");
         const uint64_t per_level = (budget - row_bytes) / (q - 1);
         const uint64_t c = per_level / 16 > 64 ? per_level / 16 - 64 : 0;
-        if (c < 4 * kTile) return Status::err(GFSM_ERR_NOMEM, "memory budget too small for the query depth");
+        if (c < 4 * kTileCand) return Status::err(GFSM_ERR_NOMEM, "memory budget too small for the query depth");
         uint64_t total = row_bytes;
         for (uint32_t k = 0; k + 1 < q; k++) {
             cap[k] = c;
             total += (c * 4 + 255) / 256 * 256 * 2 + ((c + 1) * 8 + 255) / 256 * 256;
-            chunk[k + 1] = std::min<uint64_t>(c, ctx.max_tiles * kTile);
+            chunk[k + 1] = std::min<uint64_t>(c, launch_limit());
         }
-        chunk[q - 1] = ctx.max_tiles * kTile;
+        chunk[q - 1] = ctx.max_tiles * kTileCand;
         Status st = ensure_arena(ctx, total);
         if (!st.good()) return st;
         char *p = ctx.arena.base;
@@ -358,7 +379,7 @@ struct Runner {
     Status run_roots(uint64_t rlo, uint64_t rhi)
     {
         res.stats.roots = rhi - rlo;
-        uint64_t step = std::min<uint64_t>(cap[0], ctx.max_tiles * kTile);
+        uint64_t step = std::min<uint64_t>(cap[0], launch_limit());
         for (uint64_t c = rlo; c < rhi; c += step) {
             uint64_t e = std::min(rhi, c + step);
             RootArgs a{};
@@ -369,14 +390,14 @@ struct Runner {
             a.nchk = plan.dev.nroot_chk;
             std::memcpy(a.chk_rel, plan.dev.root_chk_rel, sizeof(a.chk_rel));
             a.out_vtx = vtx[0];
-            a.out_par = par[0];
             a.out_off = off[0];
             a.cap = cap[0];
             a.ws = ctx.lane.ws;
             a.tot = ctx.lane.tot_d;
-            a.ntiles = (a.n + kTile - 1) / kTile;
+            a.ntiles = (a.n + kTileCand - 1) / kTileCand;
             if (a.ntiles > ctx.max_tiles || e > data.rel_vtx.size())
                 return Status::err(GFSM_ERR_ARG, "internal: root launch shape check failed");
+            HIP_TRY(ctx.lane.before_launch());
             kbegin();
             HIP_TRY(launch_root(a, grid_for(a.ntiles), ctx.lane.stream));
             res.stats.launches++;
@@ -399,10 +420,10 @@ struct Runner {
     {
         const bool last = (d + 1 == q);
         const uint32_t kind = last ? (rows_mode ? EXPAND_LAST_ROWS : EXPAND_LAST_COUNT) : EXPAND_MID;
-        const uint64_t tile_limit = ctx.max_tiles * kTile;
+        const uint64_t tile_limit = launch_limit();
         uint64_t first = true;
         for (uint64_t c = g0; c < g1;) {
-            const uint64_t e = std::min(g1, c + std::max<uint64_t>(kTile, std::min(chunk[d], tile_limit)));
+            const uint64_t e = std::min(g1, c + std::max<uint64_t>(kTileCand, std::min(chunk[d], tile_limit)));
             ExpandArgs a{};
             a.g = dg.view;
             a.dp = plan.dev.depth[d];
@@ -431,12 +452,16 @@ struct Runner {
             }
             a.ws = ctx.lane.ws;
             a.tot = ctx.lane.tot_d;
-            a.ntiles = (e - c + kTile - 1) / kTile;
+            a.ntiles = (e - c + kTileCand - 1) / kTileCand;
             // Launch-shape checks: the workspace holds every tile, the frontier
             // is non-empty and the LDS staging fits.
             if (a.ntiles == 0 || a.ntiles > ctx.max_tiles || S_prev == 0 || S_prev > cap[d - 1] ||
                 a.dp.gen_dist == 0 || a.dp.gen_dist > d || expand_lds_bytes(d, a.dp.nchk) > 150 * 1024)
                 return Status::err(GFSM_ERR_ARG, "internal: expand launch shape check failed");
+            if (max_cand && cand_total + (e - c) > max_cand)
+                return Status::err(GFSM_ERR_CAP, "candidate cap reached");
+            cand_total += e - c;
+            HIP_TRY(ctx.lane.before_launch());
             kbegin();
             HIP_TRY(launch_expand(a, grid_for(a.ntiles), ctx.lane.stream));
             res.stats.launches++;
@@ -456,14 +481,14 @@ struct Runner {
             if (keep > limit) {
                 // The level did not fit: discard this launch and halve the range.
                 res.stats.overflows++;
-                chunk[d] = std::max<uint64_t>(kTile, (e - c) / 2);
+                chunk[d] = std::max<uint64_t>(kTileCand, (e - c) / 2);
                 continue;
             }
             res.stats.cand[d] += e - c;
             res.stats.level[d] += keep;
             if (keep) {
                 const double est = (double)limit * (double)(e - c) / (double)keep * 0.7;
-                chunk[d] = (uint64_t)std::max<double>(kTile, std::min<double>(est, (double)tile_limit));
+                chunk[d] = (uint64_t)std::max<double>(kTileCand, std::min<double>(est, (double)tile_limit));
             } else {
                 chunk[d] = std::min<uint64_t>(chunk[d] * 4, tile_limit);
             }
@@ -495,7 +520,7 @@ Status selftest_scan(Ctx &ctx, const uint32_t *vals, uint64_t n, uint64_t *prefi
     HIP_TRY(hipSetDevice(ctx.device));
     totals[0] = totals[1] = 0;
     if (n == 0) return Status::ok();
-    const uint64_t ntiles = (n + kTile - 1) / kTile;
+    const uint64_t ntiles = (n + kTileCand - 1) / kTileCand;
     if (ntiles > ctx.max_tiles) return Status::err(GFSM_ERR_ARG, "too many values for one launch");
     uint32_t *dv = nullptr, *di = nullptr;
     uint64_t *dp = nullptr;
@@ -511,6 +536,7 @@ Status selftest_scan(Ctx &ctx, const uint32_t *vals, uint64_t n, uint64_t *prefi
     if (e == hipSuccess) e = hipMemcpy(dv, vals, n * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(di, 0xFF, n * 4);
     const uint32_t grid = (uint32_t)std::min<uint64_t>(ntiles, (uint64_t)std::max<uint32_t>(ctx.num_cus, 1) * 8);
+    if (e == hipSuccess) e = ctx.lane.before_launch();
     if (e == hipSuccess) e = launch_scan_test(dv, n, dp, di, ctx.lane.ws, ctx.lane.tot_d, grid, ctx.lane.stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx.lane.stream);
     if (e == hipSuccess) e = hipMemcpy(prefix, dp, n * 8, hipMemcpyDeviceToHost);
@@ -547,6 +573,7 @@ Status match_plain(Ctx &ctx, const HostGraph &data, const HostGraph &query, cons
     if (!st.good()) return st;
 
     Runner run(ctx, data, *dg, plan, o.flags, o.max_rows, res);
+    run.max_cand = o.max_cand;
     st = run.carve(o.mem_budget ? o.mem_budget : ctx.budget);
     if (!st.good()) return st;
     uint64_t rlo, rhi;
@@ -556,9 +583,12 @@ Status match_plain(Ctx &ctx, const HostGraph &data, const HostGraph &query, cons
     HIP_TRY(hipMemsetAsync(L.tot_d, 0, sizeof(Totals), L.stream));
     HIP_TRY(hipEventRecord(L.ev[0], L.stream));
     st = run.run_roots(rlo, rhi);
+    Status capped = Status::ok();
     if (!st.good()) {
         (void)hipStreamSynchronize(L.stream);
-        return st;
+        if (st.code != GFSM_ERR_CAP) return st;
+        capped = st;  // work cap: report what was done so far
+        res.status = GFSM_ERR_CAP;
     }
     HIP_TRY(hipEventRecord(L.ev[1], L.stream));
     st = run.sync_totals();
@@ -591,7 +621,7 @@ Status match_plain(Ctx &ctx, const HostGraph &data, const HostGraph &query, cons
     for (uint32_t k = 1; k < q; k++) res.stats.checks[k] = plan.dev.depth[k].nchk;
     res.stats.alg_bytes = alg_bytes(plan, res.stats, o.flags);
     res.stats.host_ms = now_ms() - t0;
-    return Status::ok();
+    return capped;
 }
 
 }  // namespace gfsm

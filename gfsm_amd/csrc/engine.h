@@ -31,7 +31,11 @@ struct Lane {
     Totals *tot_h = nullptr;  // pinned
     hipEvent_t ev[2] = {nullptr, nullptr};
     hipEvent_t kev[2] = {nullptr, nullptr};
+    uint64_t launches = 0;  // chained-scan launches issued on this lane
     Status init(uint64_t max_tiles);
+    // Call before every chained-scan launch: clears the tile words when the
+    // 16-bit launch epoch is about to repeat.
+    hipError_t before_launch();
     void destroy();
 };
 

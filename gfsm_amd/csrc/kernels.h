@@ -21,16 +21,18 @@ struct DevGraphView {
     uint32_t nent;            // entries in nbr (bounds guard)
 };
 
-// Chained-scan workspace of one launch lane (one stream): per-tile status
-// words and (kept, candidates) aggregates / inclusive prefixes, plus the
-// control block {ticket, done, epoch}.
-struct alignas(16) Pair64 {
-    uint64_t k, c;
-};
+// Chained-scan workspace of one launch lane (one stream).  Each tile
+// publishes two self-describing 64-bit words -- kept nodes and next-depth
+// candidates -- packed as [63:62] status (0 none, 1 aggregate, 2 inclusive
+// prefix), [61:46] launch epoch, [45:0] value.  One word carries status and
+// value together, so relaxed agent-scope atomics suffice: no release/acquire
+// fences, which on a multi-XCD part would write back / invalidate L2.
+// ctl = {ticket, done, epoch, unused}.
+constexpr uint32_t kEpochBits = 16;
+constexpr uint64_t kValueMask = (1ull << 46) - 1;
 struct ScanWS {
-    uint32_t *flag;   // [max_tiles]  (epoch << 2) | {0 none, 1 aggregate, 2 inclusive}
-    Pair64 *agg;      // [max_tiles]
-    Pair64 *inc;      // [max_tiles]
+    uint64_t *wk;     // [max_tiles] kept-node words
+    uint64_t *wc;     // [max_tiles] candidate words
     uint32_t *ctl;    // [4]
     uint64_t max_tiles;
 };
@@ -83,7 +85,7 @@ struct RootArgs {
     uint64_t n;
     uint32_t nchk;             // self-loop relations of the root
     uint32_t chk_rel[GFSM_MAXCHK];
-    uint32_t *out_vtx, *out_par;
+    uint32_t *out_vtx;
     uint64_t *out_off;
     uint64_t cap;
     ScanWS ws;
@@ -91,7 +93,9 @@ struct RootArgs {
     uint64_t ntiles;
 };
 
-constexpr uint32_t kTile = 256;  // candidates per tile = threads per workgroup
+constexpr uint32_t kTile = 256;   // threads per workgroup
+constexpr uint32_t kItems = 2;    // candidates per thread per tile
+constexpr uint32_t kTileCand = kTile * kItems;  // candidates per tile
 
 // LDS bytes the expand kernel needs for depth d with `nchk` ancestor checks.
 size_t expand_lds_bytes(uint32_t d, uint32_t nchk);
@@ -106,5 +110,7 @@ hipError_t launch_scan_test(const uint32_t *vals, uint64_t n, uint64_t *out, uin
 
 // Zero the control block of a workspace (device kernel; safe under capture).
 hipError_t launch_ws_reset(ScanWS ws, Totals *tot, hipStream_t s);
+// Zero every tile word (needed once per 2^kEpochBits launches on a lane).
+hipError_t launch_ws_clear(ScanWS ws, hipStream_t s);
 
 }  // namespace gfsm

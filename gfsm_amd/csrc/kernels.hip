@@ -1,19 +1,20 @@
 // kernels.hip -- CDNA4 (gfx950) kernels of the forest-based matcher.
 //
 // One launch per (depth, candidate range).  A workgroup of 256 threads
-// (4 wave64s) owns a tile of 256 consecutive candidates of the depth:
-//   1. it finds the frontier M-nodes whose candidate ranges cover the tile
-//      (binary search over their exclusive offsets),
-//   2. stages, per frontier node, the placed vertices (one walk up the forest
-//      of parent pointers), the generator's segment base and every checked
-//      ancestor's segment bounds in LDS,
-//   3. gives each thread one candidate: a coalesced read of the generator
-//      segment, injectivity against the staged ancestors, and a binary search
-//      of each checked segment,
-//   4. compacts the survivors with a wave64 ballot/mbcnt count and a DPP
-//      prefix sum of their next-depth candidate counts, chained across
-//      workgroups by a decoupled look-back, so that the next depth's offsets
-//      come out of the same pass (no separate scan kernels).
+// (4 wave64s) owns a tile of kTileCand consecutive candidates of the depth
+// (kItems per thread, striped so that loads stay coalesced):
+//   1. one wave finds the frontier M-nodes whose candidate ranges cover the
+//      tile (a 32-ary search per tile end over their exclusive offsets),
+//   2. per frontier node the workgroup stages in LDS the placed vertices (one
+//      walk up the forest of parent pointers), the generator segment's base
+//      and the bounds of every checked ancestor segment,
+//   3. every candidate is read coalesced from its generator segment, checked
+//      for injectivity against the staged ancestors and binary-searched in
+//      each checked segment,
+//   4. survivors are compacted with wave64 ballot/mbcnt counts and DPP prefix
+//      sums of their next-depth candidate counts, chained across workgroups
+//      by a decoupled look-back, so that the next depth's offsets come out of
+//      the same pass.
 // This replaces hnGen + hnCheck + CUB ExclusiveSum + hnRemove + the degree
 // scan of one reference depth (HnGPU.cu:114-262, 309-349, 371-439).
 #include <hip/hip_runtime.h>
@@ -109,114 +110,122 @@ DEV bool seg_contains(const uint32_t *__restrict__ nbr, uint32_t beg, uint32_t e
 }
 
 // ---------------------------------------------------------------------------
-// chained scan across workgroups (decoupled look-back)
+// chained scan across workgroups (decoupled look-back over packed words)
 // ---------------------------------------------------------------------------
 
-DEV uint32_t ld_acq(const uint32_t *p) { return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT); }
-DEV void st_rel(uint32_t *p, uint32_t v) { __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT); }
-DEV uint64_t ld_rlx(const uint64_t *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
-DEV void st_rlx(uint64_t *p, uint64_t v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+DEV uint64_t ld_word(const uint64_t *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+DEV void st_word(uint64_t *p, uint64_t v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+DEV uint64_t pack(uint64_t st, uint32_t ep, uint64_t v)
+{
+    return (st << 62) | ((uint64_t)(ep & ((1u << kEpochBits) - 1)) << 46) | (v & kValueMask);
+}
 
-// Exclusive (kept, candidates) prefix of tile t, computed by all 64 lanes of
-// one wave: each pass reads the status of the 64 preceding tiles, waits until
+// Exclusive prefix of tile t over one word array, computed by all 64 lanes of
+// one wave: each pass reads the words of the 64 preceding tiles, waits until
 // every tile up to the nearest inclusive prefix has published, and sums.
-DEV void lookback(const ScanWS &ws, uint32_t tag, uint64_t t, uint64_t &ek, uint64_t &ec)
+DEV uint64_t lookback(const uint64_t *words, uint32_t ep, uint64_t t)
 {
     const uint32_t lane = lane_id();
-    uint64_t sk = 0, sc = 0;
+    const uint64_t epm = (uint64_t)(ep & ((1u << kEpochBits) - 1));
+    uint64_t sum = 0;
     int64_t base = (int64_t)t - 1;
     for (;;) {
         const int64_t idx = base - (int64_t)lane;
+        uint64_t w = 0, pm;
         uint32_t st;
-        uint64_t pm;
         for (;;) {
             st = 2u;  // before tile 0 the inclusive prefix is 0
+            w = 0;
             if (idx >= 0) {
-                uint32_t f = ld_acq(ws.flag + idx);
-                st = ((f & ~3u) == tag) ? (f & 3u) : 0u;
+                w = ld_word(words + idx);
+                st = (((w >> 46) & ((1u << kEpochBits) - 1)) == epm) ? (uint32_t)(w >> 62) : 0u;
             }
             pm = __ballot(st == 2u);
-            uint64_t xm = __ballot(st == 0u);
-            uint64_t upto = pm ? (pm ^ (pm - 1)) : ~0ull;
+            const uint64_t xm = __ballot(st == 0u);
+            const uint64_t upto = pm ? (pm ^ (pm - 1)) : ~0ull;
             if (!(xm & upto)) break;
             __builtin_amdgcn_s_sleep(1);
         }
-        uint64_t upto = pm ? (pm ^ (pm - 1)) : ~0ull;
-        uint64_t vk = 0, vc = 0;
-        if (((upto >> lane) & 1ull) && idx >= 0) {
-            const Pair64 *src = (st == 2u) ? ws.inc : ws.agg;
-            vk = ld_rlx(&src[idx].k);
-            vc = ld_rlx(&src[idx].c);
-        }
-        sk += wave_sum64(vk);
-        sc += wave_sum64(vc);
+        const uint64_t upto = pm ? (pm ^ (pm - 1)) : ~0ull;
+        const uint64_t v = ((upto >> lane) & 1ull) ? (w & kValueMask) : 0ull;
+        sum += wave_sum64(v);
         if (pm) break;
         base -= 64;
     }
-    ek = sk;
-    ec = sc;
+    return sum;
 }
 
 struct ScanSmem {
-    uint32_t wk[kWaves];
-    uint64_t wc[kWaves];
+    uint32_t wk[kItems][kWaves];
+    uint64_t wc[kItems][kWaves];
     uint64_t ex[2];
 };
 
-// Tile-wide exclusive prefix of (keep, cnt) continued across tiles.
-// Returns this thread's kept index and candidate offset, and the running
-// totals through this tile.
-DEV void tile_scan(bool keep, uint32_t cnt, uint64_t t, const ScanWS &ws, uint32_t tag, ScanSmem &sm,
-                   uint64_t &my_k, uint64_t &my_c, uint64_t &run_k, uint64_t &run_c)
+// Tile-wide exclusive prefix of (keep, cnt) in candidate order (item k of
+// thread x is tile position k * 256 + x), continued across tiles.  Returns
+// each item's kept index and candidate offset and the totals through this
+// tile.
+DEV void tile_scan(const bool (&keep)[kItems], const uint32_t (&cnt)[kItems], uint64_t t, const ScanWS &ws,
+                   uint32_t ep, ScanSmem &sm, uint64_t (&my_k)[kItems], uint64_t (&my_c)[kItems], uint64_t &run_k,
+                   uint64_t &run_c)
 {
     const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
-    const uint64_t m = __ballot(keep);
-    const uint32_t kpre = mask_prefix(m);
-    const uint32_t cinc = wave_incl_scan(cnt);
-    if (lane == 63) {
-        sm.wk[wave] = (uint32_t)__popcll(m);
-        sm.wc[wave] = cinc;
+    uint32_t kpre[kItems], cpre[kItems];
+#pragma unroll
+    for (uint32_t k = 0; k < kItems; k++) {
+        const uint64_t m = __ballot(keep[k]);
+        kpre[k] = mask_prefix(m);
+        const uint32_t cinc = wave_incl_scan(cnt[k]);
+        cpre[k] = cinc - cnt[k];
+        if (lane == 63) {
+            sm.wk[k][wave] = (uint32_t)__popcll(m);
+            sm.wc[k][wave] = cinc;
+        }
     }
     __syncthreads();
-    uint64_t wk_off = 0, wc_off = 0, tk = 0, tc = 0;
+    uint64_t ak = 0, ac = 0, offk[kItems], offc[kItems];
 #pragma unroll
-    for (uint32_t w = 0; w < kWaves; w++) {
-        if (w < wave) {
-            wk_off += sm.wk[w];
-            wc_off += sm.wc[w];
+    for (uint32_t k = 0; k < kItems; k++) {
+#pragma unroll
+        for (uint32_t w = 0; w < kWaves; w++) {
+            if (w == wave) {
+                offk[k] = ak;
+                offc[k] = ac;
+            }
+            ak += sm.wk[k][w];
+            ac += sm.wc[k][w];
         }
-        tk += sm.wk[w];
-        tc += sm.wc[w];
     }
     if (wave == 0) {
         uint64_t ek = 0, ec = 0;
         if (t != 0) {
             if (lane == 0) {
-                st_rlx(&ws.agg[t].k, tk);
-                st_rlx(&ws.agg[t].c, tc);
-                st_rel(ws.flag + t, tag | 1u);
+                st_word(ws.wk + t, pack(1, ep, ak));
+                st_word(ws.wc + t, pack(1, ep, ac));
             }
-            lookback(ws, tag, t, ek, ec);
+            ek = lookback(ws.wk, ep, t);
+            ec = lookback(ws.wc, ep, t);
         }
         if (lane == 0) {
-            st_rlx(&ws.inc[t].k, ek + tk);
-            st_rlx(&ws.inc[t].c, ec + tc);
-            st_rel(ws.flag + t, tag | 2u);
+            st_word(ws.wk + t, pack(2, ep, ek + ak));
+            st_word(ws.wc + t, pack(2, ep, ec + ac));
             sm.ex[0] = ek;
             sm.ex[1] = ec;
         }
     }
     __syncthreads();
-    my_k = sm.ex[0] + wk_off + kpre;
-    my_c = sm.ex[1] + wc_off + (cinc - cnt);
-    run_k = sm.ex[0] + tk;
-    run_c = sm.ex[1] + tc;
+#pragma unroll
+    for (uint32_t k = 0; k < kItems; k++) {
+        my_k[k] = sm.ex[0] + offk[k] + kpre[k];
+        my_c[k] = sm.ex[1] + offc[k] + cpre[k];
+    }
+    run_k = sm.ex[0] + ak;
+    run_c = sm.ex[1] + ac;
 }
 
-DEV uint32_t read_tag(const ScanWS &ws)
+DEV uint32_t read_epoch(const ScanWS &ws)
 {
-    uint32_t e = __hip_atomic_load(ws.ctl + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return (e & 0x3FFFFFFFu) << 2;
+    return __hip_atomic_load(ws.ctl + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Tile ticket: tiles are handed out in launch order, so a tile only waits on
@@ -238,8 +247,35 @@ DEV void finish_launch(const ScanWS &ws)
         if (prev == gridDim.x - 1) {
             __hip_atomic_store(ws.ctl + 0, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_store(ws.ctl + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_fetch_add(ws.ctl + 2, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(ws.ctl + 2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
+    }
+}
+
+// Sum (count, digest0, digest1) over the workgroup and add it to the launch
+// totals: one set of atomics per workgroup, not per wave.
+DEV void flush_sums(uint64_t cnt, uint64_t h0, uint64_t h1, Totals *tot, uint64_t (*s_red)[kWaves])
+{
+    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+    cnt = wave_sum64(cnt);
+    h0 = wave_sum64(h0);
+    h1 = wave_sum64(h1);
+    if (lane == 0) {
+        s_red[0][wave] = cnt;
+        s_red[1][wave] = h0;
+        s_red[2][wave] = h1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t c = 0, a = 0, b = 0;
+        for (uint32_t w = 0; w < kWaves; w++) {
+            c += s_red[0][w];
+            a += s_red[1][w];
+            b += s_red[2][w];
+        }
+        if (c) atomicAdd((unsigned long long *)&tot->count, (unsigned long long)c);
+        if (a) atomicAdd((unsigned long long *)&tot->dg0, (unsigned long long)a);
+        if (b) atomicAdd((unsigned long long *)&tot->dg1, (unsigned long long)b);
     }
 }
 
@@ -251,29 +287,38 @@ __global__ __launch_bounds__(kTile) void k_root(RootArgs a)
 {
     __shared__ ScanSmem sm;
     __shared__ uint64_t s_tile;
-    const uint32_t tag = read_tag(a.ws);
+    const uint32_t ep = read_epoch(a.ws);
     for (;;) {
         const uint64_t t = next_tile(a.ws, &s_tile);
         if (t >= a.ntiles) break;
-        const uint64_t c = t * kTile + threadIdx.x;
-        bool keep = false;
-        uint32_t v = 0, cnt = 0;
-        if (c < a.n) {
-            v = a.cand_vtx[c];
-            cnt = a.cand_cnt[c];
-            keep = cnt > 0;
-            for (uint32_t r = 0; r < a.nchk && keep; r++) {
-                uint2 sg = seg_lookup(a.g, v, a.chk_rel[r]);
-                keep = seg_contains(a.g.nbr, sg.x, sg.y, v);
+        bool keep[kItems];
+        uint32_t v[kItems], cnt[kItems];
+#pragma unroll
+        for (uint32_t k = 0; k < kItems; k++) {
+            const uint64_t c = t * kTileCand + k * kTile + threadIdx.x;
+            keep[k] = false;
+            v[k] = 0;
+            cnt[k] = 0;
+            if (c < a.n) {
+                v[k] = a.cand_vtx[c];
+                cnt[k] = a.cand_cnt[c];
+                bool ok = cnt[k] > 0;
+                for (uint32_t r = 0; r < a.nchk && ok; r++) {
+                    uint2 sg = seg_lookup(a.g, v[k], a.chk_rel[r]);
+                    ok = seg_contains(a.g.nbr, sg.x, sg.y, v[k]);
+                }
+                keep[k] = ok;
+                if (!ok) cnt[k] = 0;
             }
-            if (!keep) cnt = 0;
         }
-        uint64_t k, off, rk, rc;
-        tile_scan(keep, cnt, t, a.ws, tag, sm, k, off, rk, rc);
-        if (keep && k < a.cap) {
-            a.out_vtx[k] = v;
-            a.out_par[k] = 0u;
-            a.out_off[k] = off;
+        uint64_t kk[kItems], off[kItems], rk, rc;
+        tile_scan(keep, cnt, t, a.ws, ep, sm, kk, off, rk, rc);
+#pragma unroll
+        for (uint32_t k = 0; k < kItems; k++) {
+            if (keep[k] && kk[k] < a.cap) {
+                a.out_vtx[kk[k]] = v[k];
+                a.out_off[kk[k]] = off[k];
+            }
         }
         if (t == a.ntiles - 1 && threadIdx.x == 0) {
             a.tot->keep = rk;
@@ -288,6 +333,25 @@ __global__ __launch_bounds__(kTile) void k_root(RootArgs a)
 // one depth: expand + filter + compact + next-depth offsets
 // ---------------------------------------------------------------------------
 
+// Largest i in [0, S) with off[i] <= key, found by half a wave (32 lanes,
+// `sub` = lane within the half) in 32-ary steps; off[0] = 0 <= key.
+DEV uint64_t frontier_search(const uint64_t *__restrict__ off, uint64_t S, uint64_t key, uint32_t sub)
+{
+    uint64_t lo = 0, hi = S;
+    while (hi - lo > 1) {
+        const uint64_t n = hi - lo;
+        const uint64_t stride = (n + 31) / 32;
+        const uint64_t pos = lo + (uint64_t)sub * stride;
+        const bool ok = pos < hi && off[pos] <= key;
+        const uint64_t m = __ballot(ok);
+        const uint32_t half = (uint32_t)((lane_id() < 32) ? (m & 0xFFFFFFFFull) : (m >> 32));
+        const uint32_t top = 31u - __builtin_clz(half | 1u);  // lane 0 of the half always qualifies
+        lo = lo + (uint64_t)top * stride;
+        hi = (lo + stride < hi) ? lo + stride : hi;
+    }
+    return lo;
+}
+
 DEV size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
 
 template <uint32_t KIND>
@@ -297,40 +361,34 @@ __global__ __launch_bounds__(kTile) void k_expand(ExpandArgs a)
     __shared__ ScanSmem sm;
     __shared__ uint64_t s_tile;
     __shared__ uint64_t s_node[2];
-    constexpr uint32_t T = kTile;
+    __shared__ uint64_t s_red[3][kWaves];
+    constexpr uint32_t T = kTile, TC = kTileCand;
     const uint32_t d = a.d;
     const uint32_t tid = threadIdx.x;
 
-    uint32_t *s_first = reinterpret_cast<uint32_t *>(smem);                     // [T+1]
-    uint64_t *s_seg = reinterpret_cast<uint64_t *>(smem + align16((T + 1) * 4)); // [T]
-    uint2 *s_chk = reinterpret_cast<uint2 *>(s_seg + T);                        // [nchk][T]
-    uint32_t *s_anc = reinterpret_cast<uint32_t *>(s_chk + a.dp.nchk * T);      // [d][T]
-    uint32_t *s_nxt = s_anc + d * T;                                            // [T]
+    // LDS staging, sized for TC frontier nodes (a tile never spans more).
+    uint32_t *s_first = reinterpret_cast<uint32_t *>(smem);                          // [TC+1]
+    uint32_t *s_seg = reinterpret_cast<uint32_t *>(smem + align16((TC + 1) * 4));    // [TC]
+    uint2 *s_chk = reinterpret_cast<uint2 *>(s_seg + TC);                            // [nchk][TC]
+    uint32_t *s_anc = reinterpret_cast<uint32_t *>(s_chk + a.dp.nchk * TC);          // [d][TC]
+    uint32_t *s_nxt = s_anc + d * TC;                                                // [TC]
 
-    const uint32_t tag = read_tag(a.ws);
+    const uint32_t ep = read_epoch(a.ws);
+    uint64_t acc_cnt = 0, acc_h0 = 0, acc_h1 = 0;
     for (;;) {
         const uint64_t t = next_tile(a.ws, &s_tile);
         if (t >= a.ntiles) break;
-        const uint64_t c0 = a.g_begin + t * T;
-        const uint64_t c1 = (c0 + T < a.g_end) ? c0 + T : a.g_end;
+        const uint64_t c0 = a.g_begin + t * TC;
+        const uint64_t c1 = (c0 + TC < a.g_end) ? c0 + TC : a.g_end;
         const uint32_t n = (uint32_t)(c1 - c0);
 
-        // Frontier nodes covering [c0, c1): last node starting at or before
-        // each end (offsets are strictly increasing: every kept node has
-        // at least one candidate).
-        if (tid < 2) {
-            const uint64_t key = tid ? c1 - 1 : c0;
-            uint64_t lo = 0, len = a.S_prev;
-            while (len > 1) {
-                uint64_t h = len >> 1;
-                if (a.off_prev[lo + h] <= key) {
-                    lo += h;
-                    len -= h;
-                } else {
-                    len = h;
-                }
-            }
-            s_node[tid] = lo;
+        // Frontier nodes covering [c0, c1): offsets are strictly increasing
+        // (every kept node has at least one candidate).
+        if (tid < 64) {
+            const uint32_t sub = tid & 31;
+            const uint64_t key = (tid < 32) ? c0 : c1 - 1;
+            const uint64_t i = frontier_search(a.off_prev, a.S_prev, key, sub);
+            if (sub == 0) s_node[tid >> 5] = i;
         }
         __syncthreads();
         const uint64_t ilo = s_node[0];
@@ -342,72 +400,88 @@ __global__ __launch_bounds__(kTile) void k_expand(ExpandArgs a)
             const uint64_t o = a.off_prev[i];
             s_first[j] = (o > c0) ? (uint32_t)(o - c0) : 0u;
             uint32_t v = a.lv.vtx[d - 1][i];
-            s_anc[(d - 1) * T + j] = v;
+            s_anc[(d - 1) * TC + j] = v;
             uint32_t p = (d >= 2) ? a.lv.par[d - 1][i] : 0u;
             for (int k = (int)d - 2; k >= 0; k--) {
                 v = a.lv.vtx[k][p];
-                s_anc[k * T + j] = v;
+                s_anc[k * TC + j] = v;
                 if (k > 0) p = a.lv.par[k][p];
             }
-            const uint32_t gp = s_anc[(d - a.dp.gen_dist) * T + j];
+            const uint32_t gp = s_anc[(d - a.dp.gen_dist) * TC + j];
             const uint2 sg = seg_lookup(a.g, gp, a.dp.gen_rel);
-            s_seg[j] = (uint64_t)sg.x - o;
+            // nbr index of tile position x inside this node's range: base + x
+            s_seg[j] = sg.x + (uint32_t)(c0 - o);
             for (uint32_t r = 0; r < a.dp.nchk; r++) {
                 const uint32_t dist = a.dp.chk_dist[r];
-                if (dist) s_chk[r * T + j] = seg_lookup(a.g, s_anc[(d - dist) * T + j], a.dp.chk_rel[r]);
+                if (dist) s_chk[r * TC + j] = seg_lookup(a.g, s_anc[(d - dist) * TC + j], a.dp.chk_rel[r]);
             }
             if (KIND == EXPAND_MID && a.dp.nxt_dist > 1) {
-                const uint2 ns = seg_lookup(a.g, s_anc[(d + 1 - a.dp.nxt_dist) * T + j], a.dp.nxt_rel);
+                const uint2 ns = seg_lookup(a.g, s_anc[(d + 1 - a.dp.nxt_dist) * TC + j], a.dp.nxt_rel);
                 s_nxt[j] = ns.y - ns.x;
             }
         }
         if (tid == 0) s_first[nf] = n;
         __syncthreads();
 
-        bool keep = false;
-        uint32_t cnt = 0, w = 0, j = 0;
-        if (tid < n) {
-            uint32_t lo = 0, len = nf;
-            while (len > 1) {
-                uint32_t h = len >> 1;
-                if (s_first[lo + h] <= tid) {
-                    lo += h;
-                    len -= h;
-                } else {
-                    len = h;
-                }
-            }
-            j = lo;
-            const uint64_t pos = s_seg[j] + c0 + tid;
-            bool ok = pos < a.g.nent;  // offsets and segments always agree; never read past the array
-            w = ok ? a.g.nbr[pos] : 0u;
-            for (uint32_t k = 0; k < d; k++) ok &= (s_anc[k * T + j] != w);
-            for (uint32_t r = 0; r < a.dp.nchk && ok; r++) {
-                const uint2 sg = a.dp.chk_dist[r] ? s_chk[r * T + j] : seg_lookup(a.g, w, a.dp.chk_rel[r]);
-                ok = seg_contains(a.g.nbr, sg.x, sg.y, w);
-            }
-            if (KIND == EXPAND_MID) {
-                if (ok) {
-                    if (a.dp.nxt_dist == 1) {
-                        const uint2 ns = seg_lookup(a.g, w, a.dp.nxt_rel);
-                        cnt = ns.y - ns.x;
+        bool keep[kItems];
+        uint32_t cnt[kItems], w[kItems], jj[kItems];
+#pragma unroll
+        for (uint32_t k = 0; k < kItems; k++) {
+            const uint32_t x = k * T + tid;
+            keep[k] = false;
+            cnt[k] = 0;
+            w[k] = 0;
+            jj[k] = 0;
+            if (x < n) {
+                uint32_t lo = 0, len = nf;
+                while (len > 1) {
+                    const uint32_t h = len >> 1;
+                    if (s_first[lo + h] <= x) {
+                        lo += h;
+                        len -= h;
                     } else {
-                        cnt = s_nxt[j];
+                        len = h;
                     }
-                    keep = cnt > 0;
                 }
-            } else {
-                keep = ok;
+                const uint32_t j = lo;
+                jj[k] = j;
+                const uint32_t pos = s_seg[j] + x;
+                bool ok = pos < a.g.nent;  // offsets and segments always agree; never read past the array
+                const uint32_t wv = ok ? a.g.nbr[pos] : 0u;
+                w[k] = wv;
+                for (uint32_t kk = 0; kk < d; kk++) ok &= (s_anc[kk * TC + j] != wv);
+                for (uint32_t r = 0; r < a.dp.nchk && ok; r++) {
+                    const uint2 sg = a.dp.chk_dist[r] ? s_chk[r * TC + j] : seg_lookup(a.g, wv, a.dp.chk_rel[r]);
+                    ok = seg_contains(a.g.nbr, sg.x, sg.y, wv);
+                }
+                if (KIND == EXPAND_MID) {
+                    if (ok) {
+                        uint32_t c;
+                        if (a.dp.nxt_dist == 1) {
+                            const uint2 ns = seg_lookup(a.g, wv, a.dp.nxt_rel);
+                            c = ns.y - ns.x;
+                        } else {
+                            c = s_nxt[j];
+                        }
+                        cnt[k] = c;
+                        keep[k] = c > 0;
+                    }
+                } else {
+                    keep[k] = ok;
+                }
             }
         }
 
         if (KIND == EXPAND_MID) {
-            uint64_t k, off, rk, rc;
-            tile_scan(keep, keep ? cnt : 0u, t, a.ws, tag, sm, k, off, rk, rc);
-            if (keep && k < a.cap) {
-                a.out_vtx[k] = w;
-                a.out_par[k] = (uint32_t)(ilo + j);
-                a.out_off[k] = off;
+            uint64_t kx[kItems], off[kItems], rk, rc;
+            tile_scan(keep, cnt, t, a.ws, ep, sm, kx, off, rk, rc);
+#pragma unroll
+            for (uint32_t k = 0; k < kItems; k++) {
+                if (keep[k] && kx[k] < a.cap) {
+                    a.out_vtx[kx[k]] = w[k];
+                    a.out_par[kx[k]] = (uint32_t)(ilo + jj[k]);
+                    a.out_off[kx[k]] = off[k];
+                }
             }
             if (t == a.ntiles - 1 && tid == 0) {
                 a.tot->keep = rk;
@@ -416,47 +490,47 @@ __global__ __launch_bounds__(kTile) void k_expand(ExpandArgs a)
             }
         } else {
             if (KIND == EXPAND_LAST_ROWS) {
-                uint64_t k, off, rk, rc;
-                tile_scan(keep, 0u, t, a.ws, tag, sm, k, off, rk, rc);
-                if (keep && k < a.cap) {
-                    uint32_t *row = a.rows + k * a.q;
-                    for (uint32_t kk = 0; kk < d; kk++) row[a.col[kk]] = s_anc[kk * T + j];
-                    row[a.col[d]] = w;
+                uint64_t kx[kItems], off[kItems], rk, rc;
+                const uint32_t zero[kItems] = {};
+                tile_scan(keep, zero, t, a.ws, ep, sm, kx, off, rk, rc);
+#pragma unroll
+                for (uint32_t k = 0; k < kItems; k++) {
+                    if (keep[k] && kx[k] < a.cap) {
+                        uint32_t *row = a.rows + kx[k] * a.q;
+                        for (uint32_t kk = 0; kk < d; kk++) row[a.col[kk]] = s_anc[kk * TC + jj[k]];
+                        row[a.col[d]] = w[k];
+                    }
                 }
                 if (t == a.ntiles - 1 && tid == 0) a.tot->keep = rk;
             } else {
-                const uint64_t m = __ballot(keep);
-                if (lane_id() == 0 && m) atomicAdd((unsigned long long *)&a.tot->count, (unsigned long long)__popcll(m));
+#pragma unroll
+                for (uint32_t k = 0; k < kItems; k++) acc_cnt += keep[k] ? 1u : 0u;
             }
             if (a.digest) {
-                uint64_t h0 = 0, h1 = 0;
-                if (keep) {
+#pragma unroll
+                for (uint32_t k = 0; k < kItems; k++) {
+                    if (!keep[k]) continue;
                     uint64_t x0 = 0x9E3779B97F4A7C15ull, x1 = 0xD1B54A32D192ED03ull;
                     for (uint32_t c = 0; c < a.q; c++) {
                         const uint32_t dc = a.dcol[c];
-                        const uint32_t val = (dc == d) ? w : s_anc[dc * T + j];
-                        const uint64_t x = (uint64_t)val | ((uint64_t)c << 32);
-                        x0 = mix64(x0 ^ x);
-                        x1 = mix64(x1 ^ x);
+                        const uint32_t val = (dc == d) ? w[k] : s_anc[dc * TC + jj[k]];
+                        const uint64_t xv = (uint64_t)val | ((uint64_t)c << 32);
+                        x0 = mix64(x0 ^ xv);
+                        x1 = mix64(x1 ^ xv);
                     }
-                    h0 = x0;
-                    h1 = x1;
-                }
-                h0 = wave_sum64(h0);
-                h1 = wave_sum64(h1);
-                if (lane_id() == 0) {
-                    atomicAdd((unsigned long long *)&a.tot->dg0, (unsigned long long)h0);
-                    atomicAdd((unsigned long long *)&a.tot->dg1, (unsigned long long)h1);
+                    acc_h0 += x0;
+                    acc_h1 += x1;
                 }
             }
             __syncthreads();  // LDS staging is rewritten by the next tile
         }
     }
+    if (KIND != EXPAND_MID) flush_sums(acc_cnt, acc_h0, acc_h1, a.tot, s_red);
     finish_launch(a.ws);
 }
 
 // ---------------------------------------------------------------------------
-// chained-scan self test and workspace reset
+// chained-scan self test and workspace maintenance
 // ---------------------------------------------------------------------------
 
 __global__ __launch_bounds__(kTile) void k_scan_test(const uint32_t *vals, uint64_t n, uint64_t *out,
@@ -464,16 +538,26 @@ __global__ __launch_bounds__(kTile) void k_scan_test(const uint32_t *vals, uint6
 {
     __shared__ ScanSmem sm;
     __shared__ uint64_t s_tile;
-    const uint32_t tag = read_tag(ws);
+    const uint32_t ep = read_epoch(ws);
     for (;;) {
         const uint64_t t = next_tile(ws, &s_tile);
         if (t >= ntiles) break;
-        const uint64_t i = t * kTile + threadIdx.x;
-        const uint32_t v = i < n ? vals[i] : 0u;
-        uint64_t k, off, rk, rc;
-        tile_scan(v != 0, v, t, ws, tag, sm, k, off, rk, rc);
-        if (i < n) out[i] = off;
-        if (v) idx[k] = (uint32_t)i;
+        bool keep[kItems];
+        uint32_t v[kItems];
+#pragma unroll
+        for (uint32_t k = 0; k < kItems; k++) {
+            const uint64_t i = t * kTileCand + k * kTile + threadIdx.x;
+            v[k] = i < n ? vals[i] : 0u;
+            keep[k] = v[k] != 0;
+        }
+        uint64_t kx[kItems], off[kItems], rk, rc;
+        tile_scan(keep, v, t, ws, ep, sm, kx, off, rk, rc);
+#pragma unroll
+        for (uint32_t k = 0; k < kItems; k++) {
+            const uint64_t i = t * kTileCand + k * kTile + threadIdx.x;
+            if (i < n) out[i] = off[k];
+            if (keep[k]) idx[kx[k]] = (uint32_t)i;
+        }
         if (t == ntiles - 1 && threadIdx.x == 0) {
             tot->keep = rk;
             tot->cnt = rc;
@@ -488,18 +572,27 @@ __global__ void k_ws_reset(ScanWS ws, Totals *tot)
     if (tot && threadIdx.x < sizeof(Totals) / 8) reinterpret_cast<uint64_t *>(tot)[threadIdx.x] = 0ull;
 }
 
+__global__ void k_ws_clear(ScanWS ws)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < ws.max_tiles;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        ws.wk[i] = 0;
+        ws.wc[i] = 0;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // launch wrappers
 // ---------------------------------------------------------------------------
 
 size_t expand_lds_bytes(uint32_t d, uint32_t nchk)
 {
-    const size_t T = kTile;
-    size_t b = ((T + 1) * 4 + 15) & ~size_t(15);
-    b += T * 8;              // s_seg
-    b += (size_t)nchk * T * 8; // s_chk
-    b += (size_t)d * T * 4;  // s_anc
-    b += T * 4;              // s_nxt
+    const size_t TC = kTileCand;
+    size_t b = ((TC + 1) * 4 + 15) & ~size_t(15);  // s_first
+    b += TC * 4;                                   // s_seg
+    b += (size_t)nchk * TC * 8;                    // s_chk
+    b += (size_t)d * TC * 4;                       // s_anc
+    b += TC * 4;                                   // s_nxt
     return b;
 }
 
@@ -536,7 +629,7 @@ hipError_t launch_root(const RootArgs &a, uint32_t grid, hipStream_t s)
 hipError_t launch_scan_test(const uint32_t *vals, uint64_t n, uint64_t *out, uint32_t *idx, ScanWS ws,
                             Totals *tot, uint32_t grid, hipStream_t s)
 {
-    const uint64_t ntiles = (n + kTile - 1) / kTile;
+    const uint64_t ntiles = (n + kTileCand - 1) / kTileCand;
     hipLaunchKernelGGL(k_scan_test, dim3(grid), dim3(kTile), 0, s, vals, n, out, idx, ws, tot, ntiles);
     return hipGetLastError();
 }
@@ -544,6 +637,12 @@ hipError_t launch_scan_test(const uint32_t *vals, uint64_t n, uint64_t *out, uin
 hipError_t launch_ws_reset(ScanWS ws, Totals *tot, hipStream_t s)
 {
     hipLaunchKernelGGL(k_ws_reset, dim3(1), dim3(64), 0, s, ws, tot);
+    return hipGetLastError();
+}
+
+hipError_t launch_ws_clear(ScanWS ws, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ws_clear, dim3(1024), dim3(256), 0, s, ws);
     return hipGetLastError();
 }
 

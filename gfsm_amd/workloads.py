@@ -1,0 +1,116 @@
+"""The five benchmark workloads, built by the seeded generators of libgfsm.
+
+Every graph and query is a pure function of the seeds below, so tests, the
+benchmark and the CPU baseline see identical inputs.  Seeds follow
+"config number x 1000 + index".
+
+  1  Erdos-Renyi n=2000, average degree 8, 4 vertex labels (uniform), 1 edge
+     label; one 5-vertex random-walk query with every induced edge.
+  2  R-MAT scale 18, edge factor 8, (a,b,c,d) = (.57,.19,.19,.05), 16 vertex
+     labels Zipf(1.0), 1 edge label; 20 queries of 8 vertices (10 sparse =
+     walk tree + 1 extra edge, 10 dense = all induced edges).
+  3  1400 x 1400 grid, 5% of edges deleted, diagonals = 2% of the grid edge
+     count added, unlabeled; queries 4-cycle, 6-cycle, 7-path, 3x3 patch.
+  4  R-MAT scale 22, edge factor 8, 32 vertex labels Zipf(1.2), 4 edge
+     labels; 256 queries of 4..16 vertices, half sparse, half dense.
+  5  R-MAT scale 23, edge factor 16, 64 vertex labels Zipf(1.0), 4 edge
+     labels; a stream of 10,000 queries of 4..24 vertices (4 + geometric,
+     mean 8), Poisson arrivals.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import MODE_REF, MODE_VLABEL, WALK_DENSE, WALK_SPARSE, Graph
+
+RMAT_ABC = (0.57, 0.19, 0.19)
+
+
+def config1_data():
+    return Graph.er(2000, 8.0, nvlabels=4, nelabels=1, seed=1000, mode=MODE_VLABEL)
+
+
+def config1_queries(data):
+    return [Graph.walk_query(data, 5, kind=WALK_DENSE, seed=1001)]
+
+
+def config2_data(scale=18):
+    return Graph.rmat(scale, 8, *RMAT_ABC, nvlabels=16, zipf_s=1.0, nelabels=1, seed=2000, mode=MODE_VLABEL)
+
+
+def config2_queries(data, n=20):
+    qs = []
+    for i in range(n):
+        kind = WALK_SPARSE if i < n // 2 else WALK_DENSE
+        qs.append(Graph.walk_query(data, 8, kind=kind, seed=2001 + i))
+    return qs
+
+
+def config3_data(side=1400):
+    return Graph.grid(side, side, del_frac=0.05, diag_frac=0.02, seed=3000, mode=MODE_REF)
+
+
+def _undirected(n, edges):
+    src, dst = [], []
+    for a, b in edges:
+        src += [a, b]
+        dst += [b, a]
+    return Graph.from_edges(n, src, dst, [1] * len(src), vlabel=[1] * n, mode=MODE_REF)
+
+
+def cycle_query(k):
+    return _undirected(k, [(i, (i + 1) % k) for i in range(k)])
+
+
+def path_query(k):
+    return _undirected(k, [(i, i + 1) for i in range(k - 1)])
+
+
+def grid_patch_query(r=3, c=3):
+    e = []
+    for i in range(r):
+        for j in range(c):
+            v = i * c + j
+            if j + 1 < c:
+                e.append((v, v + 1))
+            if i + 1 < r:
+                e.append((v, v + c))
+    return _undirected(r * c, e)
+
+
+def config3_queries():
+    return {"C4": cycle_query(4), "C6": cycle_query(6), "P7": path_query(7), "grid3x3": grid_patch_query()}
+
+
+def config4_data(scale=22):
+    return Graph.rmat(scale, 8, *RMAT_ABC, nvlabels=32, zipf_s=1.2, nelabels=4, seed=4000, mode=MODE_VLABEL)
+
+
+def config4_queries(data, n=256):
+    rng = np.random.default_rng(4000)
+    sizes = rng.integers(4, 17, size=n)
+    return [Graph.walk_query(data, int(sizes[i]), kind=WALK_SPARSE if i % 2 else WALK_DENSE, seed=4001 + i)
+            for i in range(n)]
+
+
+def config5_data(scale=23):
+    return Graph.rmat(scale, 16, *RMAT_ABC, nvlabels=64, zipf_s=1.0, nelabels=4, seed=5000, mode=MODE_VLABEL)
+
+
+def config5_sizes(n=10000):
+    rng = np.random.default_rng(5000)
+    # 4 + geometric on {0, 1, ...} with mean 4, capped at 24: mean size 8
+    g = rng.geometric(0.2, size=n) - 1
+    return np.minimum(4 + g, 24).astype(int)
+
+
+def config5_queries(data, n=10000):
+    sizes = config5_sizes(n)
+    return [Graph.walk_query(data, int(sizes[i]), kind=WALK_SPARSE if i % 2 else WALK_DENSE, seed=5001 + i)
+            for i in range(n)]
+
+
+def poisson_arrivals(n, rate, seed=5999):
+    """Open-loop arrival times (seconds) of n requests at `rate` per second."""
+    rng = np.random.default_rng(seed)
+    return np.cumsum(rng.exponential(1.0 / rate, size=n))

@@ -84,6 +84,8 @@ typedef struct gfsm_match_opts {
     uint32_t tile;              /* candidates per workgroup tile (0 = default)                    */
     uint64_t max_rows;          /* cap on materialised rows (0 = no cap)                          */
     uint64_t mem_budget;        /* device bytes for intermediate levels (0 = context default)     */
+    uint64_t max_cand;          /* stop after this many candidates (0 = no cap): the call returns */
+                                /* GFSM_ERR_CAP and a partial result whose status says so         */
 } gfsm_match_opts;
 
 typedef struct gfsm_stats {

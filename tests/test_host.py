@@ -1,0 +1,169 @@
+"""Host-side logic of libgfsm on the CPU: the C ABI surface, the t/v/e parser,
+the relation-partitioned layout, the query planner (against the oracle's
+restatement of the reference compiler) and the workload generators.  No
+device calls."""
+import json
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import gfsm_amd as G
+import oracle as O
+from helpers import to_oracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def test_library_exports_every_declared_symbol():
+    header = open(os.path.join(ROOT, "include", "gfsm.h")).read()
+    declared = set(re.findall(r"\b(gfsm_\w+)\s*\(", header))
+    assert len(declared) > 30
+    out = subprocess.run(["nm", "-D", "--defined-only", G.library_path()], capture_output=True, text=True,
+                         check=True).stdout
+    exported = set(re.findall(r" T (gfsm_\w+)", out))
+    assert declared <= exported, declared - exported
+    G.lib()  # loads; every declared function has a ctypes prototype
+    assert G.lib().gfsm_version().startswith(b"gfsm")
+
+
+def test_parser_matches_oracle_and_error_codes(tmp_path):
+    for name in os.listdir(os.path.join(GOLD, "queries")):
+        p = os.path.join(GOLD, "queries", name)
+        g = G.Graph.load(p)
+        o = O.Edges.load(p)
+        V, s, d, l, vl = g.edges()
+        os_, od, ol, ovl = o.arrays()
+        assert V == o.V and np.array_equal(s, os_) and np.array_equal(d, od) and np.array_equal(l, ol)
+    cases = {
+        "missing": (None, 232), "badhead": ("t # 1\n1 0 1 1\nv 0 1\nt # -1", 233),
+        "fewv": ("t # 0\n3 0 1 1\nv 0 1\nt # -1", 234),
+        "fewe": ("t # 0\n2 3 1 1\nv 0 1\nv 1 1\ne 0 1 1\nt # -1", 234),
+        "badline": ("t # 0\n2 1 1 1\nv 0 1\nv 1 1\nq 0 1 1\nt # -1", 233),
+    }
+    for name, (text, code) in cases.items():
+        p = tmp_path / f"{name}.g"
+        if text is not None:
+            p.write_text(text)
+        with pytest.raises(G.GfsmError) as ei:
+            G.Graph.load(p)
+        assert ei.value.code == code, name
+
+
+def test_rejects_out_of_domain_entries():
+    with pytest.raises(G.GfsmError):
+        G.Graph.from_edges(3, [0], [5], [1])          # vertex id >= V
+    with pytest.raises(G.GfsmError):
+        G.Graph.from_edges(3, [0], [1], [0])          # label 0 (never remapped by the reference)
+    with pytest.raises(G.GfsmError):
+        G.Graph.from_edges(3, [0], [1], [1], vlabel=[5000, 1, 1], mode=G.MODE_VLABEL)
+
+
+def check_layout(g):
+    V, s, d, l, vl = g.edges()
+    info = g.info()
+    dir_ptr, dirs, nbr = g.layout()
+    assert dir_ptr[0] == 0 and np.all(np.diff(dir_ptr.astype(np.int64)) >= 0)
+    assert dirs[-1, 1] == len(nbr) == len(s)
+    got = []
+    for x in range(V):
+        rels = dirs[dir_ptr[x]:dir_ptr[x + 1], 0]
+        assert np.all(np.diff(rels.astype(np.int64)) > 0), "relations ascending, one segment each"
+        for j in range(dir_ptr[x], dir_ptr[x + 1]):
+            seg = nbr[dirs[j, 1]:dirs[j + 1, 1]]
+            assert len(seg) > 0 and np.all(np.diff(seg.astype(np.int64)) >= 0)
+            got += [(x, int(y), int(dirs[j, 0])) for y in seg]
+    if info["mode"] == G.MODE_REF:
+        rel = l
+    else:
+        rel = (vl[s].astype(np.uint64) << 21) | (vl[d].astype(np.uint64) << 10) | l
+    want = sorted(zip(s.tolist(), d.tolist(), [int(r) for r in rel]))
+    assert sorted(got) == want
+
+
+@pytest.mark.parametrize("mode", [G.MODE_REF, G.MODE_VLABEL])
+def test_layout_invariants(mode):
+    check_layout(G.Graph.er(300, 5.0, nvlabels=3, nelabels=3, seed=4, mode=mode))
+    check_layout(G.Graph.rmat(9, 6, nvlabels=4, nelabels=2, seed=5, mode=mode))
+
+
+def test_planner_matches_oracle_restatement():
+    qs = [G.Graph.load(os.path.join(GOLD, "queries", n)) for n in sorted(os.listdir(os.path.join(GOLD, "queries")))]
+    data = G.Graph.rmat(10, 8, nvlabels=3, nelabels=2, seed=9, mode=G.MODE_VLABEL)
+    for i in range(40):
+        qs.append(G.Graph.walk_query(data, 2 + i % 12, kind=i % 2, seed=100 + i))
+    with open(os.path.join(GOLD, "nx_fixtures.json")) as f:
+        for fx in json.load(f)["fixtures"]:
+            q = fx["query"]
+            qs.append(G.Graph.from_edges(q["V"], q["src"], q["dst"], q["elabel"], q["vlabel"], mode=fx["mode"]))
+    for q in qs:
+        col, hdr, req = q.plan()
+        mode = q.info()["mode"]
+        p = O.Plan(O.CCSR(to_oracle(q), mode)).fields()
+        assert col.tolist() == p["col"].tolist()
+        assert hdr.tolist() == p["header"].tolist()
+        assert req.tolist() == p["requirements"].tolist()
+
+
+def test_planner_rejects_disconnected_query():
+    q = G.Graph.from_edges(4, [0, 1, 2, 3], [1, 0, 3, 2], [1, 1, 1, 1])
+    with pytest.raises(G.GfsmError) as ei:
+        q.plan()
+    assert ei.value.code == 1
+
+
+def test_generators_deterministic_simple_symmetric():
+    a = G.Graph.rmat(11, 8, nvlabels=8, zipf_s=1.2, nelabels=4, seed=77, mode=G.MODE_VLABEL)
+    b = G.Graph.rmat(11, 8, nvlabels=8, zipf_s=1.2, nelabels=4, seed=77, mode=G.MODE_VLABEL)
+    for x, y in zip(a.edges()[1:], b.edges()[1:]):
+        assert np.array_equal(x, y)
+    V, s, d, l, vl = a.edges()
+    assert np.all(s != d), "no self loops"
+    pairs = set(zip(s.tolist(), d.tolist()))
+    assert len(pairs) == len(s), "no parallel entries"
+    lab = dict(zip(zip(s.tolist(), d.tolist()), l.tolist()))
+    assert all((y, x) in pairs and lab[(y, x)] == lab[(x, y)] for x, y in pairs), "symmetric, same label"
+    assert vl.min() >= 1 and vl.max() <= 8 and l.min() >= 1 and l.max() <= 4
+    # Zipf: label 1 is the most frequent
+    counts = np.bincount(vl, minlength=9)[1:]
+    assert counts[0] == counts.max()
+
+
+def test_grid_generator_counts():
+    R = C = 40
+    g = G.Graph.grid(R, C, del_frac=0.05, diag_frac=0.02, seed=3)
+    e0 = R * (C - 1) + C * (R - 1)
+    want = e0 - round(0.05 * e0) + round(0.02 * e0)
+    assert g.info()["entries"] == 2 * want
+    assert g.info()["max_degree"] <= 8
+
+
+def test_walk_queries_are_induced_connected_subgraphs():
+    data = G.Graph.er(500, 6.0, nvlabels=3, nelabels=2, seed=21, mode=G.MODE_VLABEL)
+    _, ds, dd, dl, dvl = data.edges()
+    dset = set(zip(ds.tolist(), dd.tolist(), dl.tolist()))
+    for kind in (G.WALK_DENSE, G.WALK_SPARSE):
+        for seed in range(10):
+            q = G.Graph.walk_query(data, 6, kind=kind, seed=seed)
+            V, s, d, l, vl = q.edges()
+            assert V == 6
+            und = {(min(a, b), max(a, b)) for a, b in zip(s.tolist(), d.tolist())}
+            if kind == G.WALK_SPARSE:
+                assert len(und) <= V  # walk tree (V-1 edges) plus at most one extra
+            col, hdr, req = q.plan()  # connected: plans without error
+            assert sorted(col.tolist()) == list(range(6))
+
+
+def test_cli_input_errors_need_no_device(tmp_path):
+    cli = os.path.join(ROOT, "gfsm_amd", "GFSM")
+    q = os.path.join(GOLD, "queries", "square.g")
+    assert subprocess.run([cli, q, str(tmp_path / "nope.g")], capture_output=True).returncode == 232
+    bad = tmp_path / "bad.g"
+    bad.write_text("t # 0\n3 0 1 1\nv 0 1\nt # -1")
+    assert subprocess.run([cli, q, str(bad)], capture_output=True).returncode == 234
+    disc = tmp_path / "disc.g"
+    disc.write_text("t # 0\n4 4 1 1\nv 0 1\nv 1 1\nv 2 1\nv 3 1\ne 0 1 1\ne 1 0 1\ne 2 3 1\ne 3 2 1\nt # -1")
+    assert subprocess.run([cli, str(disc), q], capture_output=True).returncode == 1

@@ -101,6 +101,7 @@ def lib():
         "gfsm_graph_upload": [vp, vp],
         "gfsm_graph_free": [vp],
         "gfsm_plan_query": [vp, P(u32), P(u32), pu32, pu32, pu32],
+        "gfsm_query_roots": [vp, vp, pu32, P(u64)],
         "gfsm_match": [vp, vp, vp, P(MatchOpts), P(vp)],
         "gfsm_result_count": [vp, P(u64)],
         "gfsm_result_rows": [vp, P(pu32), P(u32), P(u64)],
@@ -244,6 +245,15 @@ class Graph:
         return (_view(ps[0], nv.value + 1), _view(ps[1], 2 * (nd.value + 1)).reshape(-1, 2),
                 _view(ps[2], nn.value))
 
+    def roots(self, query):
+        """Ascending ids of the data vertices that can be f(root) for `query`."""
+        n = C.c_uint64()
+        _check(lib().gfsm_query_roots(self._h, query._h, None, C.byref(n)), "roots")
+        ids = np.zeros(max(1, n.value), np.uint32)
+        _check(lib().gfsm_query_roots(self._h, query._h, ids.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                      C.byref(n)), "roots")
+        return ids[: n.value]
+
     def plan(self):
         """Reference-layout plan: (col, header[3q], requirements[2*maxval*q])."""
         q, mv = C.c_uint32(), C.c_uint32()
@@ -289,10 +299,10 @@ class Match:
 
 
 def _opts(rows=False, sorted=False, digest=False, profile=False, root_lo=0, root_hi=0, max_rows=0,
-          mem_budget=0, max_cand=0):
+          mem_budget=0, max_cand=0, tile=0):
     flags = (ROWS if rows else 0) | (SORTED if sorted else 0) | (DIGEST if digest else 0) | \
         (PROFILE if profile else 0)
-    return MatchOpts(flags, root_lo, root_hi, 0, max_rows, mem_budget, max_cand)
+    return MatchOpts(flags, root_lo, root_hi, tile, max_rows, mem_budget, max_cand)
 
 
 class Context:

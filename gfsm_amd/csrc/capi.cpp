@@ -169,6 +169,32 @@ gfsm_status gfsm_plan_query(const gfsm_graph *query, uint32_t *q, uint32_t *maxv
     GUARD_END
 }
 
+gfsm_status gfsm_query_roots(const gfsm_graph *data, const gfsm_graph *query, uint32_t *ids, uint64_t *n)
+{
+    GUARD_BEGIN
+    if (!data || !query || !n) return fail(Status::err(GFSM_ERR_ARG, "null argument"));
+    Plan p;
+    Status st = plan_query(*query->g, p);
+    if (!st.good()) return fail(st);
+    const HostGraph &g = *data->g;
+    if (p.q == 1) {
+        uint64_t k = 0;
+        for (uint32_t x = 0; x < g.V; x++)
+            if (g.degree(x)) {
+                if (ids) ids[k] = x;
+                k++;
+            }
+        *n = k;
+        return ok();
+    }
+    uint64_t lo, hi;
+    root_range(g, p, 0, 0, lo, hi);
+    *n = hi - lo;
+    if (ids && hi > lo) std::memcpy(ids, g.rel_vtx.data() + lo, (hi - lo) * sizeof(uint32_t));
+    return ok();
+    GUARD_END
+}
+
 gfsm_status gfsm_match(gfsm_ctx *ctx, const gfsm_graph *data, const gfsm_graph *query, const gfsm_match_opts *opts,
                        gfsm_result **r)
 {

@@ -82,6 +82,12 @@ void Lane::destroy()
     *this = Lane{};
 }
 
+Tuning::Tuning()
+{
+    for (auto &k : items)
+        for (auto &v : k) v = kItems;
+}
+
 Status ctx_open(int device, uint64_t budget, Ctx *&out)
 {
     out = nullptr;
@@ -114,6 +120,7 @@ static void free_mirror(DevGraph *dg)
     (void)hipFree(dg->dir);
     (void)hipFree(dg->rel_vtx);
     (void)hipFree(dg->rel_cnt);
+    (void)hipFree(dg->rel_beg);
     delete dg;
 }
 
@@ -196,11 +203,12 @@ Status upload_graph(Ctx &ctx, const HostGraph &g, DevGraph *&out)
     if (st.good()) st = dev_copy(dg->dir, dir_flat, dg->bytes);
     if (st.good()) st = dev_copy(dg->rel_vtx, g.rel_vtx, dg->bytes);
     if (st.good()) st = dev_copy(dg->rel_cnt, g.rel_cnt, dg->bytes);
+    if (st.good()) st = dev_copy(dg->rel_beg, g.rel_beg, dg->bytes);
     if (!st.good()) {
         free_mirror(dg);
         return st;
     }
-    dg->view = DevGraphView{dg->nbr, dg->dir_ptr, dg->dir, dg->rel_vtx, dg->rel_cnt, g.V, (uint32_t)g.nent()};
+    dg->view = DevGraphView{dg->nbr, dg->dir_ptr, dg->dir, g.V, (uint32_t)g.nent()};
     g.dev[&ctx] = dg;
     ctx.graphs.insert(&g);
     out = dg;
@@ -269,7 +277,8 @@ Status match_single_vertex(const HostGraph &data, const Plan &plan, const gfsm_m
 namespace {
 
 // One plain match in flight: level buffers carved from the arena, the lane,
-// and the depth-first chunk recursion.
+// and the depth-first chunk recursion.  Level k (nodes with k+1 placed
+// vertices) holds rows of width k+1, candidate offsets and generator begins.
 struct Runner {
     Ctx &ctx;
     const HostGraph &data;
@@ -280,33 +289,53 @@ struct Runner {
     Result &res;
     uint32_t q = 0;
     bool rows_mode = false, digest = false, prof = false;
-    uint32_t *vtx[GFSM_MAXQ] = {}, *par[GFSM_MAXQ] = {};
+    uint32_t *vec[GFSM_MAXQ] = {}, *gbeg[GFSM_MAXQ] = {};
     uint64_t *off[GFSM_MAXQ] = {};
     uint64_t cap[GFSM_MAXQ] = {};
     uint64_t chunk[GFSM_MAXQ] = {};
     uint32_t *rows_d = nullptr;
     uint64_t rows_cap = 0;
     uint32_t grid_cap = 0;
-    uint64_t max_cand = 0;    // work cap (0 = none)
-    uint64_t cand_total = 0;  // candidates launched so far
+    uint64_t max_cand = 0;      // work cap (0 = none)
+    uint64_t cand_total = 0;    // candidates launched so far
+    uint32_t forced_items = 0;  // from gfsm_match_opts.tile (0 = tuned)
 
     Runner(Ctx &c, const HostGraph &d, const DevGraph &g, const Plan &p, uint32_t f, uint64_t mr, Result &r)
         : ctx(c), data(d), dg(g), plan(p), flags(f), max_rows(mr), res(r) {}
+
+    // Does the expand of depth d stage the next generator's segment per node?
+    // (Only when the next generator is an older ancestor, not the new vertex.)
+    uint32_t need_nxt(uint32_t kind, uint32_t d) const
+    {
+        return (kind == EXPAND_MID && plan.dev.depth[d].nxt_dist > 1) ? 1u : 0u;
+    }
+
+    // Candidates per thread for one expand launch: the tuned (or forced)
+    // value, reduced while the LDS staging would exceed 64 KB (two
+    // workgroups per CU).
+    uint32_t pick_items(uint32_t kind, uint32_t d) const
+    {
+        uint32_t it = forced_items ? forced_items : ctx.tune.items[kind][d];
+        while (it > 1 && expand_lds_bytes(plan.dev.depth[d].nchk, need_nxt(kind, d), it) > 64 * 1024) it >>= 1;
+        return it;
+    }
 
     // Candidates per launch: the workspace holds max_tiles tiles, and the
     // launch's next-depth candidate total (at most candidates x max degree)
     // must fit the 46-bit value field of the scan words.
     uint64_t launch_limit() const
     {
-        const uint64_t by_ws = ctx.max_tiles * kTileCand;
+        const uint64_t by_ws = ctx.max_tiles * kTile;  // enough tiles at any tile size
         const uint64_t by_val = kValueMask / std::max<uint64_t>(1, data.max_degree);
-        return std::max<uint64_t>(kTileCand, std::min(by_ws, by_val));
+        return std::max<uint64_t>(kTile * 8, std::min(by_ws, by_val));
     }
 
     uint32_t grid_for(uint64_t ntiles) const
     {
         return (uint32_t)std::min<uint64_t>(ntiles, grid_cap);
     }
+
+    static uint64_t round256(uint64_t b) { return (b + 255) / 256 * 256; }
 
     Status carve(uint64_t budget)
     {
@@ -315,36 +344,39 @@ struct Runner {
         digest = (flags & GFSM_DIGEST) && !rows_mode;  // rows mode: digest from the host rows
         prof = flags & GFSM_PROFILE;
         grid_cap = std::max<uint32_t>(ctx.num_cus, 1) * 8;
+        const uint64_t min_nodes = 8 * kTile;  // a level must hold one launch tile's survivors
         uint64_t row_bytes = 0;
         if (rows_mode) {
-            rows_cap = std::max<uint64_t>(budget / 4 / (4ull * q), kTileCand);
-            if (max_rows) rows_cap = std::min<uint64_t>(rows_cap, std::max<uint64_t>(max_rows, kTileCand));
-            row_bytes = (rows_cap * q * 4 + 255) & ~255ull;
+            rows_cap = std::max<uint64_t>(budget / 4 / (4ull * q), min_nodes);
+            if (max_rows) rows_cap = std::min<uint64_t>(rows_cap, std::max<uint64_t>(max_rows, min_nodes));
+            row_bytes = round256(rows_cap * q * 4);
         }
         if (budget <= row_bytes) return Status::err(GFSM_ERR_NOMEM, "memory budget too small");
+        // Equal byte share per level; a node of level k costs 4(k+1) + 12 bytes.
         const uint64_t per_level = (budget - row_bytes) / (q - 1);
-        const uint64_t c = per_level / 16 > 64 ? per_level / 16 - 64 : 0;
-        if (c < 4 * kTileCand) return Status::err(GFSM_ERR_NOMEM, "memory budget too small for the query depth");
         uint64_t total = row_bytes;
         for (uint32_t k = 0; k + 1 < q; k++) {
+            const uint64_t node_bytes = 4ull * (k + 1) + 12;
+            const uint64_t c = per_level / node_bytes > 256 ? per_level / node_bytes - 256 : 0;
+            if (c < min_nodes) return Status::err(GFSM_ERR_NOMEM, "memory budget too small for the query depth");
             cap[k] = c;
-            total += (c * 4 + 255) / 256 * 256 * 2 + ((c + 1) * 8 + 255) / 256 * 256;
+            total += round256(c * 4 * (k + 1)) + round256((c + 1) * 8) + round256(c * 4);
             chunk[k + 1] = std::min<uint64_t>(c, launch_limit());
         }
-        chunk[q - 1] = ctx.max_tiles * kTileCand;
+        chunk[q - 1] = launch_limit();
         Status st = ensure_arena(ctx, total);
         if (!st.good()) return st;
         char *p = ctx.arena.base;
         auto take = [&p](uint64_t b) {
             char *r = p;
-            p += (b + 255) / 256 * 256;
+            p += round256(b);
             return r;
         };
         if (rows_mode) rows_d = reinterpret_cast<uint32_t *>(take(rows_cap * q * 4));
         for (uint32_t k = 0; k + 1 < q; k++) {
-            vtx[k] = reinterpret_cast<uint32_t *>(take(cap[k] * 4));
-            par[k] = reinterpret_cast<uint32_t *>(take(cap[k] * 4));
+            vec[k] = reinterpret_cast<uint32_t *>(take(cap[k] * 4 * (k + 1)));
             off[k] = reinterpret_cast<uint64_t *>(take((cap[k] + 1) * 8));
+            gbeg[k] = reinterpret_cast<uint32_t *>(take(cap[k] * 4));
         }
         return Status::ok();
     }
@@ -385,16 +417,18 @@ struct Runner {
             RootArgs a{};
             a.g = dg.view;
             a.cand_vtx = dg.rel_vtx + c;
+            a.cand_beg = dg.rel_beg + c;
             a.cand_cnt = dg.rel_cnt + c;
             a.n = e - c;
             a.nchk = plan.dev.nroot_chk;
             std::memcpy(a.chk_rel, plan.dev.root_chk_rel, sizeof(a.chk_rel));
-            a.out_vtx = vtx[0];
+            a.out_vec = vec[0];
             a.out_off = off[0];
+            a.out_gbeg = gbeg[0];
             a.cap = cap[0];
             a.ws = ctx.lane.ws;
             a.tot = ctx.lane.tot_d;
-            a.ntiles = (a.n + kTileCand - 1) / kTileCand;
+            a.ntiles = (a.n + kRootTile - 1) / kRootTile;
             if (a.ntiles > ctx.max_tiles || e > data.rel_vtx.size())
                 return Status::err(GFSM_ERR_ARG, "internal: root launch shape check failed");
             HIP_TRY(ctx.lane.before_launch());
@@ -421,10 +455,14 @@ struct Runner {
         const bool last = (d + 1 == q);
         const uint32_t kind = last ? (rows_mode ? EXPAND_LAST_ROWS : EXPAND_LAST_COUNT) : EXPAND_MID;
         const uint64_t tile_limit = launch_limit();
-        uint64_t first = true;
+        const uint32_t items = pick_items(kind, d);
+        const uint64_t tc = (uint64_t)kTile * items;
+        bool first = true;
         for (uint64_t c = g0; c < g1;) {
-            const uint64_t e = std::min(g1, c + std::max<uint64_t>(kTileCand, std::min(chunk[d], tile_limit)));
+            const uint64_t e = std::min(g1, c + std::max<uint64_t>(tc, std::min(chunk[d], tile_limit)));
             ExpandArgs a{};
+            a.items = items;
+            a.need_nxt = need_nxt(kind, d);
             a.g = dg.view;
             a.dp = plan.dev.depth[d];
             a.d = d;
@@ -433,18 +471,16 @@ struct Runner {
             a.digest = (last && digest) ? 1u : 0u;
             std::memcpy(a.col, plan.dev.col, sizeof(a.col));
             std::memcpy(a.dcol, plan.dev.dcol, sizeof(a.dcol));
-            a.off_prev = off[d - 1];
+            a.in_vec = vec[d - 1];
+            a.in_off = off[d - 1];
+            a.in_gbeg = gbeg[d - 1];
             a.S_prev = S_prev;
             a.g_begin = c;
             a.g_end = e;
-            for (uint32_t k = 0; k < d; k++) {
-                a.lv.vtx[k] = vtx[k];
-                a.lv.par[k] = par[k];
-            }
             if (!last) {
-                a.out_vtx = vtx[d];
-                a.out_par = par[d];
+                a.out_vec = vec[d];
                 a.out_off = off[d];
+                a.out_gbeg = gbeg[d];
                 a.cap = cap[d];
             } else if (rows_mode) {
                 a.rows = rows_d;
@@ -452,11 +488,12 @@ struct Runner {
             }
             a.ws = ctx.lane.ws;
             a.tot = ctx.lane.tot_d;
-            a.ntiles = (e - c + kTileCand - 1) / kTileCand;
+            a.ntiles = (e - c + tc - 1) / tc;
             // Launch-shape checks: the workspace holds every tile, the frontier
             // is non-empty and the LDS staging fits.
             if (a.ntiles == 0 || a.ntiles > ctx.max_tiles || S_prev == 0 || S_prev > cap[d - 1] ||
-                a.dp.gen_dist == 0 || a.dp.gen_dist > d || expand_lds_bytes(d, a.dp.nchk) > 150 * 1024)
+                a.dp.gen_dist == 0 || a.dp.gen_dist > d ||
+                expand_lds_bytes(a.dp.nchk, a.need_nxt, items) > 150 * 1024)
                 return Status::err(GFSM_ERR_ARG, "internal: expand launch shape check failed");
             if (max_cand && cand_total + (e - c) > max_cand)
                 return Status::err(GFSM_ERR_CAP, "candidate cap reached");
@@ -481,14 +518,14 @@ struct Runner {
             if (keep > limit) {
                 // The level did not fit: discard this launch and halve the range.
                 res.stats.overflows++;
-                chunk[d] = std::max<uint64_t>(kTileCand, (e - c) / 2);
+                chunk[d] = std::max<uint64_t>(tc, (e - c) / 2);
                 continue;
             }
             res.stats.cand[d] += e - c;
             res.stats.level[d] += keep;
             if (keep) {
                 const double est = (double)limit * (double)(e - c) / (double)keep * 0.7;
-                chunk[d] = (uint64_t)std::max<double>(kTileCand, std::min<double>(est, (double)tile_limit));
+                chunk[d] = (uint64_t)std::max<double>((double)tc, std::min<double>(est, (double)tile_limit));
             } else {
                 chunk[d] = std::min<uint64_t>(chunk[d] * 4, tile_limit);
             }
@@ -520,7 +557,8 @@ Status selftest_scan(Ctx &ctx, const uint32_t *vals, uint64_t n, uint64_t *prefi
     HIP_TRY(hipSetDevice(ctx.device));
     totals[0] = totals[1] = 0;
     if (n == 0) return Status::ok();
-    const uint64_t ntiles = (n + kTileCand - 1) / kTileCand;
+    const uint64_t tc = (uint64_t)kTile * kScanTestItems;
+    const uint64_t ntiles = (n + tc - 1) / tc;
     if (ntiles > ctx.max_tiles) return Status::err(GFSM_ERR_ARG, "too many values for one launch");
     uint32_t *dv = nullptr, *di = nullptr;
     uint64_t *dp = nullptr;
@@ -574,6 +612,11 @@ Status match_plain(Ctx &ctx, const HostGraph &data, const HostGraph &query, cons
 
     Runner run(ctx, data, *dg, plan, o.flags, o.max_rows, res);
     run.max_cand = o.max_cand;
+    if (o.tile) {
+        if (o.tile != 256 && o.tile != 512 && o.tile != 1024 && o.tile != 2048)
+            return Status::err(GFSM_ERR_ARG, "tile must be 0, 256, 512, 1024 or 2048 candidates");
+        run.forced_items = o.tile / kTile;
+    }
     st = run.carve(o.mem_budget ? o.mem_budget : ctx.budget);
     if (!st.good()) return st;
     uint64_t rlo, rhi;

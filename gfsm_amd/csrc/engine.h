@@ -16,7 +16,8 @@ namespace gfsm {
 struct DevGraph {
     int device = 0;
     const void *ctx = nullptr;
-    uint32_t *nbr = nullptr, *dir_ptr = nullptr, *dir = nullptr, *rel_vtx = nullptr, *rel_cnt = nullptr;
+    uint32_t *nbr = nullptr, *dir_ptr = nullptr, *dir = nullptr, *rel_vtx = nullptr, *rel_beg = nullptr,
+             *rel_cnt = nullptr;
     DevGraphView view{};
     size_t bytes = 0;
 };
@@ -45,6 +46,13 @@ struct Arena {
     uint64_t bytes = 0;
 };
 
+// Tile shapes of the expand kernel: candidates per thread (1, 2, 4, 8) per
+// kernel kind and depth.  Filled with defaults, replaced by the autotuner.
+struct Tuning {
+    uint8_t items[3][GFSM_MAXQ];
+    Tuning();
+};
+
 struct Ctx {
     int device = 0;
     uint64_t budget = 0;
@@ -52,6 +60,7 @@ struct Ctx {
     uint64_t max_tiles = 0;
     Lane lane;
     Arena arena;
+    Tuning tune;
     std::mutex mu;
     std::set<const HostGraph *> graphs;  // data graphs resident on this context
 };

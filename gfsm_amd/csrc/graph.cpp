@@ -231,6 +231,7 @@ Status build_graph(uint32_t nv, const uint32_t *vlabel, uint64_t ne, const uint3
         g->rel_off[nk] = acc;
         g->rel_vtx.resize(acc);
         g->rel_cnt.resize(acc);
+        g->rel_beg.resize(acc);
         std::vector<std::thread> ts;
         for (unsigned t = 0; t < nt; t++) ts.emplace_back([&, t] {
             uint64_t b, e;
@@ -240,6 +241,7 @@ Status build_graph(uint32_t nv, const uint32_t *vlabel, uint64_t ne, const uint3
                     uint64_t p = cnt[(size_t)kidx[j] * nt + t]++;
                     g->rel_vtx[p] = (uint32_t)v;
                     g->rel_cnt[p] = g->dir[j + 1].beg - g->dir[j].beg;
+                    g->rel_beg[p] = g->dir[j].beg;
                 }
         });
         for (auto &t : ts) t.join();

@@ -39,8 +39,9 @@ struct DevGraph;  // device mirror, owned by a context
 //     by target id inside a group (one "segment" per (vertex, relation));
 //   - dir_ptr[x] .. dir_ptr[x+1] index x's segment directory;
 //   - dir has a sentinel {~0u, nent} so that dir[j+1].beg ends segment j;
-//   - rel_keys/rel_off/rel_vtx/rel_cnt list, per relation, the vertices that
-//     own a segment of that relation and its length (root candidate lists).
+//   - rel_keys/rel_off/rel_vtx/rel_beg/rel_cnt list, per relation, the
+//     vertices that own a segment of that relation, the segment's begin and
+//     its length (root candidate lists).
 struct HostGraph {
     uint32_t mode = GFSM_MODE_REF;
     uint32_t V = 0;
@@ -51,7 +52,7 @@ struct HostGraph {
     std::vector<DirEnt> dir;
     std::vector<uint32_t> rel_keys;
     std::vector<uint64_t> rel_off;
-    std::vector<uint32_t> rel_vtx, rel_cnt;
+    std::vector<uint32_t> rel_vtx, rel_beg, rel_cnt;
     uint32_t max_degree = 0;
     uint64_t nent() const { return nbr.size(); }
     uint64_t ndir() const { return dir.empty() ? 0 : dir.size() - 1; }

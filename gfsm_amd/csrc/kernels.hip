@@ -1,20 +1,20 @@
 // kernels.hip -- CDNA4 (gfx950) kernels of the forest-based matcher.
 //
 // One launch per (depth, candidate range).  A workgroup of 256 threads
-// (4 wave64s) owns a tile of kTileCand consecutive candidates of the depth
-// (kItems per thread, striped so that loads stay coalesced):
-//   1. one wave finds the frontier M-nodes whose candidate ranges cover the
-//      tile (a 32-ary search per tile end over their exclusive offsets),
-//   2. per frontier node the workgroup stages in LDS the placed vertices (one
-//      walk up the forest of parent pointers), the generator segment's base
-//      and the bounds of every checked ancestor segment,
-//   3. every candidate is read coalesced from its generator segment, checked
-//      for injectivity against the staged ancestors and binary-searched in
-//      each checked segment,
-//   4. survivors are compacted with wave64 ballot/mbcnt counts and DPP prefix
+// (4 wave64s) owns a tile of 256 * I consecutive candidates of the depth
+// (I = 1, 2, 4 or 8 per thread, striped so that loads stay coalesced):
+//   1. one wave finds the frontier nodes whose candidate ranges cover the
+//      tile (a 32-ary search per tile end over their exclusive offsets) and
+//      marks where each node's candidates start in an LDS bitmap,
+//   2. each candidate finds its node from the bitmap (mbcnt of a wave-uniform
+//      word), reads itself coalesced from the node's generator segment, is
+//      checked for injectivity against the node's placed vertices (a row of
+//      the previous level, read through the cache) and binary-searched in the
+//      segments of the checked ancestors (staged per node in LDS),
+//   3. survivors are compacted with wave64 ballot/mbcnt counts and DPP prefix
 //      sums of their next-depth candidate counts, chained across workgroups
-//      by a decoupled look-back, so that the next depth's offsets come out of
-//      the same pass.
+//      by a decoupled look-back, so that the next level -- rows, candidate
+//      offsets and generator segment begins -- comes out of the same pass.
 // This replaces hnGen + hnCheck + CUB ExclusiveSum + hnRemove + the degree
 // scan of one reference depth (HnGPU.cu:114-262, 309-349, 371-439).
 #include <hip/hip_runtime.h>
@@ -155,9 +155,10 @@ DEV uint64_t lookback(const uint64_t *words, uint32_t ep, uint64_t t)
     return sum;
 }
 
+template <uint32_t I>
 struct ScanSmem {
-    uint32_t wk[kItems][kWaves];
-    uint64_t wc[kItems][kWaves];
+    uint32_t wk[I][kWaves];
+    uint64_t wc[I][kWaves];
     uint64_t ex[2];
 };
 
@@ -165,14 +166,14 @@ struct ScanSmem {
 // thread x is tile position k * 256 + x), continued across tiles.  Returns
 // each item's kept index and candidate offset and the totals through this
 // tile.
-DEV void tile_scan(const bool (&keep)[kItems], const uint32_t (&cnt)[kItems], uint64_t t, const ScanWS &ws,
-                   uint32_t ep, ScanSmem &sm, uint64_t (&my_k)[kItems], uint64_t (&my_c)[kItems], uint64_t &run_k,
-                   uint64_t &run_c)
+template <uint32_t I>
+DEV void tile_scan(const bool (&keep)[I], const uint32_t (&cnt)[I], uint64_t t, const ScanWS &ws, uint32_t ep,
+                   ScanSmem<I> &sm, uint64_t (&my_k)[I], uint64_t (&my_c)[I], uint64_t &run_k, uint64_t &run_c)
 {
     const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
-    uint32_t kpre[kItems], cpre[kItems];
+    uint32_t kpre[I], cpre[I];
 #pragma unroll
-    for (uint32_t k = 0; k < kItems; k++) {
+    for (uint32_t k = 0; k < I; k++) {
         const uint64_t m = __ballot(keep[k]);
         kpre[k] = mask_prefix(m);
         const uint32_t cinc = wave_incl_scan(cnt[k]);
@@ -183,9 +184,9 @@ DEV void tile_scan(const bool (&keep)[kItems], const uint32_t (&cnt)[kItems], ui
         }
     }
     __syncthreads();
-    uint64_t ak = 0, ac = 0, offk[kItems], offc[kItems];
+    uint64_t ak = 0, ac = 0, offk[I], offc[I];
 #pragma unroll
-    for (uint32_t k = 0; k < kItems; k++) {
+    for (uint32_t k = 0; k < I; k++) {
 #pragma unroll
         for (uint32_t w = 0; w < kWaves; w++) {
             if (w == wave) {
@@ -215,7 +216,7 @@ DEV void tile_scan(const bool (&keep)[kItems], const uint32_t (&cnt)[kItems], ui
     }
     __syncthreads();
 #pragma unroll
-    for (uint32_t k = 0; k < kItems; k++) {
+    for (uint32_t k = 0; k < I; k++) {
         my_k[k] = sm.ex[0] + offk[k] + kpre[k];
         my_c[k] = sm.ex[1] + offc[k] + cpre[k];
     }
@@ -228,13 +229,22 @@ DEV uint32_t read_epoch(const ScanWS &ws)
     return __hip_atomic_load(ws.ctl + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Tile ticket: tiles are handed out in launch order, so a tile only waits on
-// tiles already owned by running workgroups.
-DEV uint64_t next_tile(const ScanWS &ws, uint64_t *s_tile)
+// Tile tickets: tiles are handed out in launch order, so a tile only waits on
+// tiles already owned by running workgroups (a prefetched ticket is always
+// processed next by its owner, after its current tile).  Thread 0 keeps one
+// ticket in flight so the atomic's latency overlaps the current tile.
+DEV void tickets_init(const ScanWS &ws, uint32_t &next)
 {
-    if (threadIdx.x == 0) *s_tile = atomicAdd(ws.ctl + 0, 1u);
+    if (threadIdx.x == 0) next = atomicAdd(ws.ctl + 0, 1u);
+}
+
+DEV uint64_t tickets_take(const ScanWS &ws, uint32_t &next, uint64_t *s_tile, uint64_t ntiles)
+{
+    if (threadIdx.x == 0) *s_tile = next;
     __syncthreads();
-    return *s_tile;
+    const uint64_t t = *s_tile;
+    if (threadIdx.x == 0 && t < ntiles) next = atomicAdd(ws.ctl + 0, 1u);
+    return t;
 }
 
 // The last workgroup to finish resets the ticket and advances the epoch.
@@ -283,19 +293,22 @@ DEV void flush_sums(uint64_t cnt, uint64_t h0, uint64_t h1, Totals *tot, uint64_
 // root level (hnInit, HnGPU.cu:76-109)
 // ---------------------------------------------------------------------------
 
+template <uint32_t I>
 __global__ __launch_bounds__(kTile) void k_root(RootArgs a)
 {
-    __shared__ ScanSmem sm;
+    __shared__ ScanSmem<I> sm;
     __shared__ uint64_t s_tile;
     const uint32_t ep = read_epoch(a.ws);
+    uint32_t next = 0;
+    tickets_init(a.ws, next);
     for (;;) {
-        const uint64_t t = next_tile(a.ws, &s_tile);
+        const uint64_t t = tickets_take(a.ws, next, &s_tile, a.ntiles);
         if (t >= a.ntiles) break;
-        bool keep[kItems];
-        uint32_t v[kItems], cnt[kItems];
+        bool keep[I];
+        uint32_t v[I], cnt[I];
 #pragma unroll
-        for (uint32_t k = 0; k < kItems; k++) {
-            const uint64_t c = t * kTileCand + k * kTile + threadIdx.x;
+        for (uint32_t k = 0; k < I; k++) {
+            const uint64_t c = t * (kTile * I) + k * kTile + threadIdx.x;
             keep[k] = false;
             v[k] = 0;
             cnt[k] = 0;
@@ -311,13 +324,15 @@ __global__ __launch_bounds__(kTile) void k_root(RootArgs a)
                 if (!ok) cnt[k] = 0;
             }
         }
-        uint64_t kk[kItems], off[kItems], rk, rc;
-        tile_scan(keep, cnt, t, a.ws, ep, sm, kk, off, rk, rc);
+        uint64_t kk[I], off[I], rk, rc;
+        tile_scan<I>(keep, cnt, t, a.ws, ep, sm, kk, off, rk, rc);
 #pragma unroll
-        for (uint32_t k = 0; k < kItems; k++) {
+        for (uint32_t k = 0; k < I; k++) {
             if (keep[k] && kk[k] < a.cap) {
-                a.out_vtx[kk[k]] = v[k];
+                const uint64_t c = t * (kTile * I) + k * kTile + threadIdx.x;
+                a.out_vec[kk[k]] = v[k];
                 a.out_off[kk[k]] = off[k];
+                a.out_gbeg[kk[k]] = a.cand_beg[c];
             }
         }
         if (t == a.ntiles - 1 && threadIdx.x == 0) {
@@ -330,7 +345,7 @@ __global__ __launch_bounds__(kTile) void k_root(RootArgs a)
 }
 
 // ---------------------------------------------------------------------------
-// one depth: expand + filter + compact + next-depth offsets
+// one depth: expand + filter + compact + next level
 // ---------------------------------------------------------------------------
 
 // Largest i in [0, S) with off[i] <= key, found by half a wave (32 lanes,
@@ -352,119 +367,101 @@ DEV uint64_t frontier_search(const uint64_t *__restrict__ off, uint64_t S, uint6
     return lo;
 }
 
-DEV size_t align16(size_t x) { return (x + 15) & ~size_t(15); }
-
-template <uint32_t KIND>
+template <uint32_t KIND, uint32_t I>
 __global__ __launch_bounds__(kTile) void k_expand(ExpandArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ ScanSmem sm;
+    __shared__ ScanSmem<I> sm;
     __shared__ uint64_t s_tile;
     __shared__ uint64_t s_node[2];
     __shared__ uint64_t s_red[3][kWaves];
-    constexpr uint32_t T = kTile, TC = kTileCand;
+    constexpr uint32_t T = kTile, TC = kTile * I, NW = TC / 64;
     const uint32_t d = a.d;
     const uint32_t tid = threadIdx.x;
+    const uint32_t lane = lane_id(), wave = tid >> 6;
 
     // LDS staging, sized for TC frontier nodes (a tile never spans more).
-    uint32_t *s_first = reinterpret_cast<uint32_t *>(smem);                          // [TC+1]
-    uint32_t *s_seg = reinterpret_cast<uint32_t *>(smem + align16((TC + 1) * 4));    // [TC]
-    uint2 *s_chk = reinterpret_cast<uint2 *>(s_seg + TC);                            // [nchk][TC]
-    uint32_t *s_anc = reinterpret_cast<uint32_t *>(s_chk + a.dp.nchk * TC);          // [d][TC]
-    uint32_t *s_nxt = s_anc + d * TC;                                                // [TC]
+    // s_bits marks the tile positions where a node's candidates start: the
+    // node of position x is (set bits at positions <= x) - 1.
+    uint64_t *s_bits = reinterpret_cast<uint64_t *>(smem);                // [NW]
+    uint32_t *s_seg = reinterpret_cast<uint32_t *>(s_bits + NW);          // [TC] nbr index of tile position 0
+    uint2 *s_chk = reinterpret_cast<uint2 *>(s_seg + TC);                 // [nchk][TC] checked segments
+    uint2 *s_nxt = s_chk + a.dp.nchk * TC;                                // [TC] next generator (need_nxt)
 
     const uint32_t ep = read_epoch(a.ws);
     uint64_t acc_cnt = 0, acc_h0 = 0, acc_h1 = 0;
+    uint32_t next = 0;
+    tickets_init(a.ws, next);
     for (;;) {
-        const uint64_t t = next_tile(a.ws, &s_tile);
+        const uint64_t t = tickets_take(a.ws, next, &s_tile, a.ntiles);
         if (t >= a.ntiles) break;
         const uint64_t c0 = a.g_begin + t * TC;
         const uint64_t c1 = (c0 + TC < a.g_end) ? c0 + TC : a.g_end;
         const uint32_t n = (uint32_t)(c1 - c0);
 
-        // Frontier nodes covering [c0, c1): offsets are strictly increasing
-        // (every kept node has at least one candidate).
+        // Frontier nodes covering [c0, c1): offsets are strictly increasing.
+        if (tid < NW) s_bits[tid] = 0;
         if (tid < 64) {
             const uint32_t sub = tid & 31;
             const uint64_t key = (tid < 32) ? c0 : c1 - 1;
-            const uint64_t i = frontier_search(a.off_prev, a.S_prev, key, sub);
+            const uint64_t i = frontier_search(a.in_off, a.S_prev, key, sub);
             if (sub == 0) s_node[tid >> 5] = i;
         }
         __syncthreads();
         const uint64_t ilo = s_node[0];
         const uint32_t nf = (uint32_t)(s_node[1] - ilo + 1);
 
-        // Per frontier node: placed vertices, generator base, checked segments.
+        // Per frontier node: start bit, generator base, checked segments.
         for (uint32_t j = tid; j < nf; j += T) {
             const uint64_t i = ilo + j;
-            const uint64_t o = a.off_prev[i];
-            s_first[j] = (o > c0) ? (uint32_t)(o - c0) : 0u;
-            uint32_t v = a.lv.vtx[d - 1][i];
-            s_anc[(d - 1) * TC + j] = v;
-            uint32_t p = (d >= 2) ? a.lv.par[d - 1][i] : 0u;
-            for (int k = (int)d - 2; k >= 0; k--) {
-                v = a.lv.vtx[k][p];
-                s_anc[k * TC + j] = v;
-                if (k > 0) p = a.lv.par[k][p];
-            }
-            const uint32_t gp = s_anc[(d - a.dp.gen_dist) * TC + j];
-            const uint2 sg = seg_lookup(a.g, gp, a.dp.gen_rel);
-            // nbr index of tile position x inside this node's range: base + x
-            s_seg[j] = sg.x + (uint32_t)(c0 - o);
+            const uint64_t o = a.in_off[i];
+            const uint32_t first = (o > c0) ? (uint32_t)(o - c0) : 0u;
+            atomicOr((unsigned long long *)&s_bits[first >> 6], 1ull << (first & 63));
+            s_seg[j] = a.in_gbeg[i] + (uint32_t)(c0 - o);
+            const uint32_t *av = a.in_vec + i * d;
             for (uint32_t r = 0; r < a.dp.nchk; r++) {
                 const uint32_t dist = a.dp.chk_dist[r];
-                if (dist) s_chk[r * TC + j] = seg_lookup(a.g, s_anc[(d - dist) * TC + j], a.dp.chk_rel[r]);
+                if (dist) s_chk[r * TC + j] = seg_lookup(a.g, av[d - dist], a.dp.chk_rel[r]);
             }
-            if (KIND == EXPAND_MID && a.dp.nxt_dist > 1) {
-                const uint2 ns = seg_lookup(a.g, s_anc[(d + 1 - a.dp.nxt_dist) * TC + j], a.dp.nxt_rel);
-                s_nxt[j] = ns.y - ns.x;
-            }
+            if (KIND == EXPAND_MID && a.need_nxt) s_nxt[j] = seg_lookup(a.g, av[d + 1 - a.dp.nxt_dist], a.dp.nxt_rel);
         }
-        if (tid == 0) s_first[nf] = n;
         __syncthreads();
 
-        bool keep[kItems];
-        uint32_t cnt[kItems], w[kItems], jj[kItems];
+        bool keep[I];
+        uint32_t cnt[I], w[I], gb[I];
+        uint64_t node[I];
 #pragma unroll
-        for (uint32_t k = 0; k < kItems; k++) {
+        for (uint32_t k = 0; k < I; k++) {
             const uint32_t x = k * T + tid;
             keep[k] = false;
             cnt[k] = 0;
             w[k] = 0;
-            jj[k] = 0;
+            gb[k] = 0;
+            node[k] = ilo;
             if (x < n) {
-                uint32_t lo = 0, len = nf;
-                while (len > 1) {
-                    const uint32_t h = len >> 1;
-                    if (s_first[lo + h] <= x) {
-                        lo += h;
-                        len -= h;
-                    } else {
-                        len = h;
-                    }
-                }
-                const uint32_t j = lo;
-                jj[k] = j;
+                // word k * 4 + wave covers this wave's 64 positions of item k
+                const uint32_t wi = k * kWaves + wave;
+                const uint64_t bits = s_bits[wi];
+                uint32_t pre = 0;
+                for (uint32_t u = 0; u < wi; u++) pre += (uint32_t)__popcll(s_bits[u]);
+                const uint32_t j = pre + mask_prefix(bits) + (uint32_t)((bits >> lane) & 1ull) - 1u;
+                node[k] = ilo + j;
                 const uint32_t pos = s_seg[j] + x;
                 bool ok = pos < a.g.nent;  // offsets and segments always agree; never read past the array
                 const uint32_t wv = ok ? a.g.nbr[pos] : 0u;
                 w[k] = wv;
-                for (uint32_t kk = 0; kk < d; kk++) ok &= (s_anc[kk * TC + j] != wv);
+                const uint32_t *av = a.in_vec + node[k] * d;
+                for (uint32_t kk = 0; kk < d; kk++) ok &= (av[kk] != wv);
                 for (uint32_t r = 0; r < a.dp.nchk && ok; r++) {
                     const uint2 sg = a.dp.chk_dist[r] ? s_chk[r * TC + j] : seg_lookup(a.g, wv, a.dp.chk_rel[r]);
                     ok = seg_contains(a.g.nbr, sg.x, sg.y, wv);
                 }
                 if (KIND == EXPAND_MID) {
                     if (ok) {
-                        uint32_t c;
-                        if (a.dp.nxt_dist == 1) {
-                            const uint2 ns = seg_lookup(a.g, wv, a.dp.nxt_rel);
-                            c = ns.y - ns.x;
-                        } else {
-                            c = s_nxt[j];
-                        }
-                        cnt[k] = c;
-                        keep[k] = c > 0;
+                        const uint2 ns = a.need_nxt ? s_nxt[j] : seg_lookup(a.g, wv, a.dp.nxt_rel);
+                        cnt[k] = ns.y - ns.x;
+                        gb[k] = ns.x;
+                        keep[k] = cnt[k] > 0;
                     }
                 } else {
                     keep[k] = ok;
@@ -473,14 +470,18 @@ __global__ __launch_bounds__(kTile) void k_expand(ExpandArgs a)
         }
 
         if (KIND == EXPAND_MID) {
-            uint64_t kx[kItems], off[kItems], rk, rc;
-            tile_scan(keep, cnt, t, a.ws, ep, sm, kx, off, rk, rc);
+            uint64_t kx[I], off[I], rk, rc;
+            tile_scan<I>(keep, cnt, t, a.ws, ep, sm, kx, off, rk, rc);
+            const uint32_t wd = d + 1;
 #pragma unroll
-            for (uint32_t k = 0; k < kItems; k++) {
+            for (uint32_t k = 0; k < I; k++) {
                 if (keep[k] && kx[k] < a.cap) {
-                    a.out_vtx[kx[k]] = w[k];
-                    a.out_par[kx[k]] = (uint32_t)(ilo + jj[k]);
+                    const uint32_t *av = a.in_vec + node[k] * d;
+                    uint32_t *ov = a.out_vec + kx[k] * wd;
+                    for (uint32_t kk = 0; kk < d; kk++) ov[kk] = av[kk];
+                    ov[d] = w[k];
                     a.out_off[kx[k]] = off[k];
+                    a.out_gbeg[kx[k]] = gb[k];
                 }
             }
             if (t == a.ntiles - 1 && tid == 0) {
@@ -490,30 +491,32 @@ __global__ __launch_bounds__(kTile) void k_expand(ExpandArgs a)
             }
         } else {
             if (KIND == EXPAND_LAST_ROWS) {
-                uint64_t kx[kItems], off[kItems], rk, rc;
-                const uint32_t zero[kItems] = {};
-                tile_scan(keep, zero, t, a.ws, ep, sm, kx, off, rk, rc);
+                uint64_t kx[I], off[I], rk, rc;
+                const uint32_t zero[I] = {};
+                tile_scan<I>(keep, zero, t, a.ws, ep, sm, kx, off, rk, rc);
 #pragma unroll
-                for (uint32_t k = 0; k < kItems; k++) {
+                for (uint32_t k = 0; k < I; k++) {
                     if (keep[k] && kx[k] < a.cap) {
+                        const uint32_t *av = a.in_vec + node[k] * d;
                         uint32_t *row = a.rows + kx[k] * a.q;
-                        for (uint32_t kk = 0; kk < d; kk++) row[a.col[kk]] = s_anc[kk * TC + jj[k]];
+                        for (uint32_t kk = 0; kk < d; kk++) row[a.col[kk]] = av[kk];
                         row[a.col[d]] = w[k];
                     }
                 }
                 if (t == a.ntiles - 1 && tid == 0) a.tot->keep = rk;
             } else {
 #pragma unroll
-                for (uint32_t k = 0; k < kItems; k++) acc_cnt += keep[k] ? 1u : 0u;
+                for (uint32_t k = 0; k < I; k++) acc_cnt += keep[k] ? 1u : 0u;
             }
             if (a.digest) {
 #pragma unroll
-                for (uint32_t k = 0; k < kItems; k++) {
+                for (uint32_t k = 0; k < I; k++) {
                     if (!keep[k]) continue;
+                    const uint32_t *av = a.in_vec + node[k] * d;
                     uint64_t x0 = 0x9E3779B97F4A7C15ull, x1 = 0xD1B54A32D192ED03ull;
                     for (uint32_t c = 0; c < a.q; c++) {
                         const uint32_t dc = a.dcol[c];
-                        const uint32_t val = (dc == d) ? w[k] : s_anc[dc * TC + jj[k]];
+                        const uint32_t val = (dc == d) ? w[k] : av[dc];
                         const uint64_t xv = (uint64_t)val | ((uint64_t)c << 32);
                         x0 = mix64(x0 ^ xv);
                         x1 = mix64(x1 ^ xv);
@@ -536,25 +539,28 @@ __global__ __launch_bounds__(kTile) void k_expand(ExpandArgs a)
 __global__ __launch_bounds__(kTile) void k_scan_test(const uint32_t *vals, uint64_t n, uint64_t *out,
                                                     uint32_t *idx, ScanWS ws, Totals *tot, uint64_t ntiles)
 {
-    __shared__ ScanSmem sm;
+    constexpr uint32_t I = kScanTestItems;
+    __shared__ ScanSmem<I> sm;
     __shared__ uint64_t s_tile;
     const uint32_t ep = read_epoch(ws);
+    uint32_t next = 0;
+    tickets_init(ws, next);
     for (;;) {
-        const uint64_t t = next_tile(ws, &s_tile);
+        const uint64_t t = tickets_take(ws, next, &s_tile, ntiles);
         if (t >= ntiles) break;
-        bool keep[kItems];
-        uint32_t v[kItems];
+        bool keep[I];
+        uint32_t v[I];
 #pragma unroll
-        for (uint32_t k = 0; k < kItems; k++) {
-            const uint64_t i = t * kTileCand + k * kTile + threadIdx.x;
+        for (uint32_t k = 0; k < I; k++) {
+            const uint64_t i = t * (kTile * I) + k * kTile + threadIdx.x;
             v[k] = i < n ? vals[i] : 0u;
             keep[k] = v[k] != 0;
         }
-        uint64_t kx[kItems], off[kItems], rk, rc;
-        tile_scan(keep, v, t, ws, ep, sm, kx, off, rk, rc);
+        uint64_t kx[I], off[I], rk, rc;
+        tile_scan<I>(keep, v, t, ws, ep, sm, kx, off, rk, rc);
 #pragma unroll
-        for (uint32_t k = 0; k < kItems; k++) {
-            const uint64_t i = t * kTileCand + k * kTile + threadIdx.x;
+        for (uint32_t k = 0; k < I; k++) {
+            const uint64_t i = t * (kTile * I) + k * kTile + threadIdx.x;
             if (i < n) out[i] = off[k];
             if (keep[k]) idx[kx[k]] = (uint32_t)i;
         }
@@ -585,51 +591,63 @@ __global__ void k_ws_clear(ScanWS ws)
 // launch wrappers
 // ---------------------------------------------------------------------------
 
-size_t expand_lds_bytes(uint32_t d, uint32_t nchk)
+size_t expand_lds_bytes(uint32_t nchk, uint32_t need_nxt, uint32_t items)
 {
-    const size_t TC = kTileCand;
-    size_t b = ((TC + 1) * 4 + 15) & ~size_t(15);  // s_first
-    b += TC * 4;                                   // s_seg
-    b += (size_t)nchk * TC * 8;                    // s_chk
-    b += (size_t)d * TC * 4;                       // s_anc
-    b += TC * 4;                                   // s_nxt
+    const size_t TC = (size_t)kTile * items;
+    size_t b = TC / 64 * 8;                 // s_bits
+    b += TC * 4;                            // s_seg
+    b += (size_t)nchk * TC * 8;             // s_chk
+    if (need_nxt) b += TC * 8;              // s_nxt
     return b;
 }
 
-template <uint32_t KIND>
+template <uint32_t KIND, uint32_t I>
 static hipError_t launch_expand_t(const ExpandArgs &a, uint32_t grid, hipStream_t s)
 {
-    const size_t lds = expand_lds_bytes(a.d, a.dp.nchk);
+    const size_t lds = expand_lds_bytes(a.dp.nchk, a.need_nxt, I);
     static bool attr_set = false;
     if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_expand<KIND>),
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_expand<KIND, I>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024);
         if (e != hipSuccess) return e;
         attr_set = true;
     }
-    hipLaunchKernelGGL(k_expand<KIND>, dim3(grid), dim3(kTile), lds, s, a);
+    hipLaunchKernelGGL((k_expand<KIND, I>), dim3(grid), dim3(kTile), lds, s, a);
     return hipGetLastError();
+}
+
+template <uint32_t KIND>
+static hipError_t launch_expand_k(const ExpandArgs &a, uint32_t grid, hipStream_t s)
+{
+    switch (a.items) {
+    case 1: return launch_expand_t<KIND, 1>(a, grid, s);
+    case 2: return launch_expand_t<KIND, 2>(a, grid, s);
+    case 4: return launch_expand_t<KIND, 4>(a, grid, s);
+    case 8: return launch_expand_t<KIND, 8>(a, grid, s);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_expand(const ExpandArgs &a, uint32_t grid, hipStream_t s)
 {
     switch (a.kind) {
-    case EXPAND_MID: return launch_expand_t<EXPAND_MID>(a, grid, s);
-    case EXPAND_LAST_COUNT: return launch_expand_t<EXPAND_LAST_COUNT>(a, grid, s);
-    default: return launch_expand_t<EXPAND_LAST_ROWS>(a, grid, s);
+    case EXPAND_MID: return launch_expand_k<EXPAND_MID>(a, grid, s);
+    case EXPAND_LAST_COUNT: return launch_expand_k<EXPAND_LAST_COUNT>(a, grid, s);
+    default: return launch_expand_k<EXPAND_LAST_ROWS>(a, grid, s);
     }
 }
 
 hipError_t launch_root(const RootArgs &a, uint32_t grid, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_root, dim3(grid), dim3(kTile), 0, s, a);
+    hipLaunchKernelGGL((k_root<kRootItems>), dim3(grid), dim3(kTile), 0, s, a);
     return hipGetLastError();
 }
 
 hipError_t launch_scan_test(const uint32_t *vals, uint64_t n, uint64_t *out, uint32_t *idx, ScanWS ws,
                             Totals *tot, uint32_t grid, hipStream_t s)
 {
-    const uint64_t ntiles = (n + kTileCand - 1) / kTileCand;
+    const uint64_t tc = (uint64_t)kTile * kScanTestItems;
+    const uint64_t ntiles = (n + tc - 1) / tc;
     hipLaunchKernelGGL(k_scan_test, dim3(grid), dim3(kTile), 0, s, vals, n, out, idx, ws, tot, ntiles);
     return hipGetLastError();
 }

@@ -141,6 +141,12 @@ void        gfsm_graph_free(gfsm_graph *g);
 gfsm_status gfsm_plan_query(const gfsm_graph *query, uint32_t *q, uint32_t *maxval, uint32_t *col,
                             uint32_t *header, uint32_t *requirements);
 
+/* Root candidates of `query` in `data`: the data vertices (ascending ids) that
+ * own a segment of the depth-1 generator relation, i.e. every f(root) that can
+ * start an embedding.  Pass ids = NULL to learn n first.  (No counterpart: the
+ * reference starts from every row, HnGPU.cu:475-495.) */
+gfsm_status gfsm_query_roots(const gfsm_graph *data, const gfsm_graph *query, uint32_t *ids, uint64_t *n);
+
 /* ---- matching ------------------------------------------------------------ */
 gfsm_status gfsm_match(gfsm_ctx *ctx, const gfsm_graph *data, const gfsm_graph *query,
                        const gfsm_match_opts *opts, gfsm_result **r);

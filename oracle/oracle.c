@@ -316,6 +316,19 @@ int orc_ccsr_build(const orc_edges *g, uint32_t mode, orc_ccsr **out)
         qsort(tmp, len, sizeof(entry_pair), entry_cmp);
         for (uint32_t k = 0; k < len; k++) { c->tgt[i + 1 + k] = tmp[k].tgt; c->rel[i + 1 + k] = tmp[k].rel; }
     }
+    /* Enumeration aid: the same rows ordered by (relation, target), and a
+     * flag for rows that repeat a target (parallel entries). */
+    c->rtgt = (uint32_t *)calloc(W ? W : 1, sizeof(uint32_t));
+    c->rrel = (uint32_t *)calloc(W ? W : 1, sizeof(uint32_t));
+    c->dup = (uint8_t *)calloc(W ? W : 1, 1);
+    for (uint64_t i = 0; i < W; i += (uint64_t)c->tgt[i] + 1) {
+        uint32_t len = c->tgt[i];
+        for (uint32_t k = 1; k < len; k++)
+            if (c->tgt[i + 1 + k] == c->tgt[i + k]) c->dup[i] = 1;
+        for (uint32_t k = 0; k < len; k++) { tmp[k].tgt = c->rel[i + 1 + k]; tmp[k].rel = c->tgt[i + 1 + k]; }
+        qsort(tmp, len, sizeof(entry_pair), entry_cmp);  /* by (relation, target) */
+        for (uint32_t k = 0; k < len; k++) { c->rrel[i + 1 + k] = tmp[k].tgt; c->rtgt[i + 1 + k] = tmp[k].rel; }
+    }
     free(tmp);
     free(degreeCounts);
     free(ord);
@@ -330,6 +343,7 @@ void orc_ccsr_free(orc_ccsr *c)
     if (!c) return;
     free(c->tgt); free(c->rel); free(c->verticeLocs); free(c->invVerticeLocs);
     free(c->staggers); free(c->stagOffset); free(c->stagNorm);
+    free(c->rtgt); free(c->rrel); free(c->dup);
     free(c);
 }
 
@@ -556,6 +570,9 @@ typedef struct {
     const orc_plan *p;
     const orc_ccsr *qg;
     uint64_t max_rows;
+    uint64_t max_leaves;     /* stop after about this many leaves (0 = never) */
+    uint64_t *leaves;        /* shared leaf counter, flushed every 4096 leaves */
+    int *stop;
 } match_ctx;
 
 static void emit_leaf(const match_ctx *mc, const uint32_t *m, thread_acc *acc)
@@ -566,6 +583,15 @@ static void emit_leaf(const match_ctx *mc, const uint32_t *m, thread_acc *acc)
     uint64_t h[2];
     orc_row_hash(row, q, h);
     acc->count++;
+    if (mc->max_leaves && (acc->count & 4095) == 0) {
+        uint64_t n;
+#pragma omp atomic capture
+        n = *mc->leaves += 4096;
+        if (n >= mc->max_leaves) {
+#pragma omp atomic write
+            *mc->stop = 1;
+        }
+    }
     acc->d0 += h[0];
     acc->d1 += h[1];
     if (mc->max_rows) {
@@ -581,9 +607,33 @@ static void emit_leaf(const match_ctx *mc, const uint32_t *m, thread_acc *acc)
     }
 }
 
+/* One candidate c at depth d: hnCheck rejects it when it equals any
+ * ancestor (HnGPU.cu:223-231) or misses a requirement (243-257). */
+static inline int check_candidate(const match_ctx *mc, uint32_t d, const uint32_t *m, uint32_t c)
+{
+    const orc_ccsr *g = mc->g;
+    const orc_plan *p = mc->p;
+    uint32_t num = p->header[d * 3 + 0];
+    uint32_t roff = p->header[d * 3 + 1];
+    for (uint32_t k = 0; k < d; k++)
+        if (m[k] == c) return 0;
+    uint32_t cur = d;
+    for (uint32_t j = 0; j < num; j++) {
+        cur -= p->requirements[roff + 2 * j];
+        uint32_t anc = (cur == d) ? c : m[cur];
+        if (!req_exists(g, anc, p->requirements[roff + 2 * j + 1], c)) return 0;
+    }
+    return 1;
+}
+
 /* Depth d >= 1: hnGen takes every entry of the row of the ancestor firstQ
- * levels up (HnGPU.cu:130-175); hnCheck rejects a candidate equal to any
- * ancestor (223-231) or missing any requirement (243-257). */
+ * levels up (HnGPU.cu:130-175) and hnCheck filters them.  A candidate that
+ * passes carries, in that row, an entry with the relation of the first
+ * requirement at distance firstQ.  When the row repeats no target, that entry
+ * is the one that produced the candidate, so only the entries with that
+ * relation (a contiguous run of the (relation, target)-ordered copy) can pass,
+ * and iterating just them yields exactly the same embeddings; rows that repeat
+ * a target are iterated in full, as hnGen does. */
 static void expand(const match_ctx *mc, uint32_t d, uint32_t *m, thread_acc *acc)
 {
     const orc_ccsr *g = mc->g;
@@ -595,18 +645,35 @@ static void expand(const match_ctx *mc, uint32_t d, uint32_t *m, thread_acc *acc
     uint32_t firstQ = p->header[d * 3 + 2];
     uint32_t parent = m[d - firstQ];
     uint32_t len = g->tgt[parent];
-    for (uint32_t e = 0; e < len; e++) {
-        uint32_t c = g->tgt[parent + 1 + e];
-        int ok = 1;
-        for (uint32_t k = 0; k < d && ok; k++)
-            if (m[k] == c) ok = 0;
-        uint32_t cur = d;
-        for (uint32_t j = 0; j < num && ok; j++) {
+    if (!g->dup[parent]) {
+        uint32_t cur = d, grel = 0;
+        int found = 0;
+        for (uint32_t j = 0; j < num && !found; j++) {
             cur -= p->requirements[roff + 2 * j];
-            uint32_t anc = (cur == d) ? c : m[cur];
-            if (!req_exists(g, anc, p->requirements[roff + 2 * j + 1], c)) ok = 0;
+            if (d - cur == firstQ) { grel = p->requirements[roff + 2 * j + 1]; found = 1; }
         }
-        if (!ok) continue;
+        if (found) {
+            const uint32_t *rr = g->rrel + parent + 1, *rt = g->rtgt + parent + 1;
+            uint32_t lo = 0, hi = len;
+            while (lo < hi) {
+                uint32_t mid = lo + ((hi - lo) >> 1);
+                if (rr[mid] < grel) lo = mid + 1; else hi = mid;
+            }
+            for (uint32_t e = lo; e < len && rr[e] == grel; e++) {
+                if (mc->max_leaves && *(volatile int *)mc->stop) return;
+                uint32_t c = rt[e];
+                if (!check_candidate(mc, d, m, c)) continue;
+                m[d] = c;
+                acc->level[d]++;
+                expand(mc, d + 1, m, acc);
+            }
+            return;
+        }
+    }
+    for (uint32_t e = 0; e < len; e++) {
+        if (mc->max_leaves && *(volatile int *)mc->stop) return;
+        uint32_t c = g->tgt[parent + 1 + e];
+        if (!check_candidate(mc, d, m, c)) continue;
         m[d] = c;
         acc->level[d]++;
         expand(mc, d + 1, m, acc);
@@ -626,6 +693,13 @@ int orc_match(const orc_ccsr *g, const orc_ccsr *qg, const orc_plan *p,
               uint32_t root_lo, uint32_t root_hi, int nthreads, uint64_t max_rows,
               orc_result **out)
 {
+    return orc_match_capped(g, qg, p, root_lo, root_hi, nthreads, max_rows, 0, out);
+}
+
+int orc_match_capped(const orc_ccsr *g, const orc_ccsr *qg, const orc_plan *p,
+                     uint32_t root_lo, uint32_t root_hi, int nthreads, uint64_t max_rows,
+                     uint64_t max_leaves, orc_result **out)
+{
     *out = NULL;
     uint32_t q = p->q;
     if (q == 0 || q > 64) return ORC_ERR_QUERY;
@@ -644,7 +718,9 @@ int orc_match(const orc_ccsr *g, const orc_ccsr *qg, const orc_plan *p,
 #endif
     thread_acc *accs = (thread_acc *)calloc((size_t)nt, sizeof(thread_acc));
     for (int t = 0; t < nt; t++) accs[t].level = (uint64_t *)calloc(q, sizeof(uint64_t));
-    match_ctx mc = {g, p, qg, max_rows};
+    uint64_t leaves = 0;
+    int stop = 0;
+    match_ctx mc = {g, p, qg, max_rows, max_leaves, &leaves, &stop};
 
 #ifdef _OPENMP
 #pragma omp parallel for schedule(dynamic, 16) num_threads(nt)
@@ -655,6 +731,7 @@ int orc_match(const orc_ccsr *g, const orc_ccsr *qg, const orc_plan *p,
         tid = omp_get_thread_num();
 #endif
         thread_acc *acc = &accs[tid];
+        if (max_leaves && *(volatile int *)&stop) continue;
         uint32_t x = roots[r];
         uint32_t id = orc_offset_to_id(g, x);
         if (id < root_lo || id >= root_hi) continue;

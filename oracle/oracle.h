@@ -67,6 +67,11 @@ typedef struct orc_ccsr {
     uint64_t *staggers;      /* [maxLen] words in rows of degree d+1               */
     uint64_t *stagOffset;    /* [maxLen] staggersInfo[d].offset                    */
     uint64_t *stagNorm;      /* [maxLen] staggersInfo[d].normOffset                */
+    /* Enumeration aid (not part of the reference layout): each row's entries
+     * again, ordered by (relation, target), and a per-row flag (at the header
+     * word) set when the row repeats a target. */
+    uint32_t *rtgt, *rrel;   /* [W]                                                */
+    uint8_t *dup;            /* [W]                                                */
 } orc_ccsr;
 
 /* Compiled query: requirements / requirementHeader exactly as the reference
@@ -115,6 +120,11 @@ void orc_plan_free(orc_plan *p);
 int  orc_match(const orc_ccsr *data, const orc_ccsr *query, const orc_plan *plan,
                uint32_t root_lo, uint32_t root_hi, int nthreads, uint64_t max_rows,
                orc_result **out);
+/* Work-bounded variant for timing samples: enumeration stops once about
+ * max_leaves embeddings have been found (count is then a partial count). */
+int  orc_match_capped(const orc_ccsr *data, const orc_ccsr *query, const orc_plan *plan,
+                      uint32_t root_lo, uint32_t root_hi, int nthreads, uint64_t max_rows,
+                      uint64_t max_leaves, orc_result **out);
 void orc_result_free(orc_result *r);
 
 /* Convenience: edges -> CCSR -> plan -> match in one call. */

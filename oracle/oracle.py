@@ -29,7 +29,8 @@ class _CCSR(C.Structure):
                 ("rel", C.POINTER(C.c_uint32)), ("verticeLocs", C.POINTER(C.c_uint32)),
                 ("invVerticeLocs", C.POINTER(C.c_uint32)), ("maxLen", C.c_uint32),
                 ("staggers", C.POINTER(C.c_uint64)), ("stagOffset", C.POINTER(C.c_uint64)),
-                ("stagNorm", C.POINTER(C.c_uint64))]
+                ("stagNorm", C.POINTER(C.c_uint64)), ("rtgt", C.POINTER(C.c_uint32)),
+                ("rrel", C.POINTER(C.c_uint32)), ("dup", C.POINTER(C.c_uint8))]
 
 
 class _Plan(C.Structure):
@@ -72,6 +73,8 @@ def lib():
         L.orc_plan_free.argtypes = [P(_Plan)]
         L.orc_match.argtypes = [P(_CCSR), P(_CCSR), P(_Plan), C.c_uint32, C.c_uint32, C.c_int,
                                 C.c_uint64, P(P(_Result))]
+        L.orc_match_capped.argtypes = [P(_CCSR), P(_CCSR), P(_Plan), C.c_uint32, C.c_uint32, C.c_int,
+                                       C.c_uint64, C.c_uint64, P(P(_Result))]
         L.orc_result_free.argtypes = [P(_Result)]
         L.orc_solve.argtypes = [P(_Edges), P(_Edges), C.c_uint32, C.c_uint32, C.c_uint32, C.c_int,
                                 C.c_uint64, P(P(_Result))]
@@ -246,6 +249,16 @@ def match(data: CCSR, query: CCSR, plan: Plan, rows=False, root_lo=0, root_hi=0x
                          max_rows if rows else 0, C.byref(p))
     if rc:
         raise OracleError(rc, "match")
+    return _take_result(p)
+
+
+def match_capped(data: CCSR, query: CCSR, plan: Plan, max_leaves, root_lo=0, root_hi=0xFFFFFFFF, threads=0):
+    """Enumerate until about max_leaves embeddings are found (timing samples)."""
+    p = C.POINTER(_Result)()
+    rc = lib().orc_match_capped(data._p, query._p, plan._p, root_lo, root_hi, threads, 0, max_leaves,
+                                C.byref(p))
+    if rc:
+        raise OracleError(rc, "match_capped")
     return _take_result(p)
 
 

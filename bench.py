@@ -34,7 +34,25 @@ def parse():
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline")
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample budget")
+    p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                   help="only run an elementwise int32 kernel that reads and writes MIB MiB, 5 times "
+                        "(scales the HBM byte counters of a profiler run)")
     return p.parse_args()
+
+
+def calibrate_copy(mib):
+    """A kernel of known HBM traffic: out = in + 1 over mib MiB of int32,
+    i.e. mib MiB read and mib MiB written per launch."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.zeros(n, dtype=torch.int32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        torch.add(src, 1, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibrate_copy_mib": mib, "launches": 5, "read_bytes_per_launch": n * 4,
+                      "write_bytes_per_launch": n * 4}))
 
 
 def traffic_from_profiles():
@@ -77,6 +95,9 @@ def cpu_baseline(data, queries, windows, budget_s):
 
 def main():
     args = parse()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               f"(the multi-GPU design shards roots, see the design notes)", file=sys.stderr)

@@ -67,6 +67,18 @@ class Stats(C.Structure):
         }
 
 
+class Tuning(C.Structure):
+    _fields_ = [("items", (C.c_uint32 * 32) * 3), ("pool_words", C.c_uint32), ("wgs_per_cu", C.c_uint32),
+                ("batch_items", C.c_uint32), ("batch_pool_words", C.c_uint32), ("batch_wgs_per_cu", C.c_uint32),
+                ("batch_slots", C.c_uint32)]
+
+    def as_dict(self):
+        return {"items": [list(self.items[k]) for k in range(3)], "pool_words": self.pool_words,
+                "wgs_per_cu": self.wgs_per_cu, "batch_items": self.batch_items,
+                "batch_pool_words": self.batch_pool_words, "batch_wgs_per_cu": self.batch_wgs_per_cu,
+                "batch_slots": self.batch_slots}
+
+
 class ServeOpts(C.Structure):
     _fields_ = [("mem_budget", C.c_uint64), ("max_inflight", C.c_uint32), ("flags", C.c_uint32)]
 
@@ -111,9 +123,12 @@ def lib():
         "gfsm_result_write_fs": [vp, C.c_char_p],
         "gfsm_result_free": [vp],
         "gfsm_row_hash": [pu32, u32, P(u64)],
-        "gfsm_match_batch": [vp, vp, P(vp), u32, P(MatchOpts), P(vp)],
+        "gfsm_match_batch": [vp, vp, P(vp), u32, P(MatchOpts), u32, P(vp)],
+        "gfsm_tuning_get": [vp, P(Tuning)],
+        "gfsm_tuning_set": [vp, P(Tuning)],
+        "gfsm_autotune": [vp, vp, P(vp), pu32, pu32, u32, u32, P(Tuning)],
         "gfsm_server_start": [vp, vp, P(ServeOpts), P(vp)],
-        "gfsm_server_submit": [vp, vp, P(u64)],
+        "gfsm_server_submit": [vp, vp, P(MatchOpts), P(u64)],
         "gfsm_server_poll": [vp, u64, P(vp)],
         "gfsm_server_wait": [vp, u64, P(vp)],
         "gfsm_server_times": [vp, u64, P(C.c_double)],
@@ -354,13 +369,55 @@ class Context:
         _check(rc, "match")
         return Match(r)
 
-    def match_batch(self, data: Graph, queries, **kw):
-        o = _opts(**kw)
+    def match_batch(self, data: Graph, queries, windows=None, caps=None, **kw):
+        """Batched matching: every query of the list in flight together.
+        windows: optional per-query (root_lo, root_hi); caps: optional
+        per-query max_cand.  Returns one Match per query; check each Match's
+        status (5 = capped, partial count)."""
         n = len(queries)
+        if windows is None and caps is None:
+            os_ = (MatchOpts * 1)(_opts(**kw))
+        else:
+            wins = windows if windows is not None else [(0, 0)] * n
+            cps = caps if caps is not None else [kw.pop("max_cand", 0)] * n
+            os_ = (MatchOpts * n)(*[_opts(root_lo=lo, root_hi=hi, max_cand=c, **kw)
+                                    for (lo, hi), c in zip(wins, cps)])
         hs = (C.c_void_p * n)(*[q._h.value for q in queries])
         rs = (C.c_void_p * n)()
-        _check(lib().gfsm_match_batch(self._h, data._h, hs, n, C.byref(o), rs), "match_batch")
+        _check(lib().gfsm_match_batch(self._h, data._h, hs, n, os_, len(os_), rs), "match_batch")
         return [Match(C.c_void_p(rs[i])) for i in range(n)]
+
+    def tuning(self) -> dict:
+        t = Tuning()
+        _check(lib().gfsm_tuning_get(self._h, C.byref(t)), "tuning_get")
+        return t.as_dict()
+
+    def set_tuning(self, **kw):
+        """Change launch shapes: items (int for all kinds/depths, or 3x32 lists),
+        pool_words, wgs_per_cu, batch_items, batch_pool_words, batch_wgs_per_cu,
+        batch_slots."""
+        t = Tuning()
+        _check(lib().gfsm_tuning_get(self._h, C.byref(t)), "tuning_get")
+        for k, v in kw.items():
+            if k == "items":
+                for a in range(3):
+                    for d in range(32):
+                        t.items[a][d] = v if isinstance(v, int) else v[a][d]
+            else:
+                setattr(t, k, v)
+        _check(lib().gfsm_tuning_set(self._h, C.byref(t)), "tuning_set")
+
+    def autotune(self, data: Graph, queries, windows=None, which=3) -> dict:
+        """Time the sample under each candidate launch shape; keep the fastest."""
+        n = len(queries)
+        hs = (C.c_void_p * n)(*[q._h.value for q in queries])
+        lo = hi = None
+        if windows is not None:
+            lo = (C.c_uint32 * n)(*[w[0] for w in windows])
+            hi = (C.c_uint32 * n)(*[w[1] for w in windows])
+        t = Tuning()
+        _check(lib().gfsm_autotune(self._h, data._h, hs, lo, hi, n, which, C.byref(t)), "autotune")
+        return t.as_dict()
 
 
 class Server:
@@ -373,9 +430,11 @@ class Server:
         _check(lib().gfsm_server_start(ctx._h, data._h, C.byref(o), C.byref(self._h)), "server_start")
         self._queries = {}
 
-    def submit(self, query: Graph) -> int:
+    def submit(self, query: Graph, **kw) -> int:
+        """Queue a query; kw: root_lo, root_hi, max_cand, digest (see Context.match)."""
         t = C.c_uint64()
-        _check(lib().gfsm_server_submit(self._h, query._h, C.byref(t)), "submit")
+        o = _opts(**kw)
+        _check(lib().gfsm_server_submit(self._h, query._h, C.byref(o), C.byref(t)), "submit")
         self._queries[t.value] = query  # keep the query alive while in flight
         return t.value
 

@@ -273,12 +273,14 @@ void gfsm_result_free(gfsm_result *r) { delete r; }
 void gfsm_row_hash(const uint32_t *row, uint32_t width, uint64_t h[2]) { row_hash(row, width, h); }
 
 gfsm_status gfsm_match_batch(gfsm_ctx *ctx, const gfsm_graph *data, const gfsm_graph *const *queries, uint32_t n,
-                             const gfsm_match_opts *opts, gfsm_result **results)
+                             const gfsm_match_opts *opts, uint32_t nopts, gfsm_result **results)
 {
     GUARD_BEGIN
     if (!ctx || !data || (n && (!queries || !results))) return fail(Status::err(GFSM_ERR_ARG, "null argument"));
-    gfsm_match_opts o{};
-    if (opts) o = *opts;
+    if (opts && nopts != 1 && nopts != n) return fail(Status::err(GFSM_ERR_ARG, "nopts must be 1 or n"));
+    std::vector<gfsm_match_opts> os;
+    if (opts) os.assign(opts, opts + nopts);
+    else os.assign(1, gfsm_match_opts{});
     std::vector<const HostGraph *> qs(n);
     for (uint32_t i = 0; i < n; i++) {
         if (!queries[i]) return fail(Status::err(GFSM_ERR_ARG, "null query"));
@@ -286,12 +288,80 @@ gfsm_status gfsm_match_batch(gfsm_ctx *ctx, const gfsm_graph *data, const gfsm_g
         results[i] = nullptr;
     }
     std::vector<Result> rs;
-    Status st = match_batch(*ctx->c, *data->g, qs, o, rs);
+    Status st = match_batch(*ctx->c, *data->g, qs, os, rs);
     if (!st.good()) return fail(st);
     for (uint32_t i = 0; i < n; i++) {
         results[i] = new gfsm_result;
         results[i]->r = std::move(rs[i]);
     }
+    return ok();
+    GUARD_END
+}
+
+gfsm_status gfsm_tuning_get(gfsm_ctx *ctx, gfsm_tuning *t)
+{
+    GUARD_BEGIN
+    if (!ctx || !t) return fail(Status::err(GFSM_ERR_ARG, "null argument"));
+    const Tuning &u = ctx->c->tune;
+    for (int k = 0; k < 3; k++)
+        for (uint32_t d = 0; d < GFSM_MAX_Q; d++) t->items[k][d] = u.items[k][d];
+    t->pool_words = u.pool_words;
+    t->wgs_per_cu = u.wgs_per_cu;
+    t->batch_items = u.batch_items;
+    t->batch_pool_words = u.batch_pool_words;
+    t->batch_wgs_per_cu = u.batch_wgs_per_cu;
+    t->batch_slots = u.batch_slots;
+    return ok();
+    GUARD_END
+}
+
+static bool valid_items(uint32_t v) { return v == 1 || v == 2 || v == 4 || v == 8; }
+
+gfsm_status gfsm_tuning_set(gfsm_ctx *ctx, const gfsm_tuning *t)
+{
+    GUARD_BEGIN
+    if (!ctx || !t) return fail(Status::err(GFSM_ERR_ARG, "null argument"));
+    for (int k = 0; k < 3; k++)
+        for (uint32_t d = 0; d < GFSM_MAX_Q; d++)
+            if (!valid_items(t->items[k][d]) || expand_lds_bytes(t->items[k][d], t->pool_words) > kMaxDynLds)
+                return fail(Status::err(GFSM_ERR_ARG, "items must be 1, 2, 4 or 8 and fit the LDS with the pool"));
+    if (!valid_items(t->batch_items) || expand_lds_bytes(t->batch_items, t->batch_pool_words) > kMaxDynLds)
+        return fail(Status::err(GFSM_ERR_ARG, "batch items must be 1, 2, 4 or 8 and fit the LDS with the pool"));
+    if (t->wgs_per_cu < 1 || t->wgs_per_cu > 32 || t->batch_wgs_per_cu < 1 || t->batch_wgs_per_cu > 32)
+        return fail(Status::err(GFSM_ERR_ARG, "workgroups per CU must be 1..32"));
+    if (t->batch_slots < 1 || t->batch_slots > kMaxSlots)
+        return fail(Status::err(GFSM_ERR_ARG, "batch slots must be 1..1024"));
+    std::lock_guard<std::mutex> lk(ctx->c->mu);
+    Tuning &u = ctx->c->tune;
+    for (int k = 0; k < 3; k++)
+        for (uint32_t d = 0; d < GFSM_MAX_Q; d++) u.items[k][d] = (uint8_t)t->items[k][d];
+    u.pool_words = t->pool_words;
+    u.wgs_per_cu = t->wgs_per_cu;
+    u.batch_items = t->batch_items;
+    u.batch_pool_words = t->batch_pool_words;
+    u.batch_wgs_per_cu = t->batch_wgs_per_cu;
+    u.batch_slots = t->batch_slots;
+    return ok();
+    GUARD_END
+}
+
+gfsm_status gfsm_autotune(gfsm_ctx *ctx, const gfsm_graph *data, const gfsm_graph *const *queries,
+                          const uint32_t *root_lo, const uint32_t *root_hi, uint32_t n, uint32_t which,
+                          gfsm_tuning *t)
+{
+    GUARD_BEGIN
+    if (!ctx || !data || !n || !queries) return fail(Status::err(GFSM_ERR_ARG, "null argument"));
+    std::vector<const HostGraph *> qs(n);
+    std::vector<uint32_t> lo(n, 0), hi(n, 0);
+    for (uint32_t i = 0; i < n; i++) {
+        if (!queries[i]) return fail(Status::err(GFSM_ERR_ARG, "null query"));
+        qs[i] = queries[i]->g;
+        if (root_lo) lo[i] = root_lo[i];
+        if (root_hi) hi[i] = root_hi[i];
+    }
+    Status st = autotune(*ctx->c, *data->g, qs, lo, hi, which);
+    if (!st.good()) return fail(st);
+    if (t) return gfsm_tuning_get(ctx, t);
     return ok();
     GUARD_END
 }
@@ -310,11 +380,12 @@ gfsm_status gfsm_server_start(gfsm_ctx *ctx, const gfsm_graph *data, const gfsm_
     GUARD_END
 }
 
-gfsm_status gfsm_server_submit(gfsm_server *srv, const gfsm_graph *query, uint64_t *ticket)
+gfsm_status gfsm_server_submit(gfsm_server *srv, const gfsm_graph *query, const gfsm_match_opts *opts,
+                               uint64_t *ticket)
 {
     GUARD_BEGIN
     if (!srv || !query || !ticket) return fail(Status::err(GFSM_ERR_ARG, "null argument"));
-    Status st = server_submit(*reinterpret_cast<Server *>(srv), *query->g, *ticket);
+    Status st = server_submit(*reinterpret_cast<Server *>(srv), *query->g, opts, *ticket);
     return st.good() ? ok() : fail(st);
     GUARD_END
 }

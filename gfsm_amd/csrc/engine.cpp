@@ -128,6 +128,7 @@ void ctx_close(Ctx *ctx)
 {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
+    ctx->drop_batch_cache();
     {
         std::lock_guard<std::mutex> lk(g_dev_mu);
         for (const HostGraph *g : ctx->graphs) {
@@ -141,6 +142,7 @@ void ctx_close(Ctx *ctx)
     }
     ctx->lane.destroy();
     if (ctx->arena.base) (void)hipFree(ctx->arena.base);
+    if (ctx->batch_arena.base) (void)hipFree(ctx->batch_arena.base);
     delete ctx;
 }
 
@@ -149,27 +151,30 @@ void release_graph_mirrors(const HostGraph &g)
     std::lock_guard<std::mutex> lk(g_dev_mu);
     for (auto &kv : g.dev) {
         Ctx *c = (Ctx *)const_cast<void *>(kv.first);
+        if (c->batch_graph == &g) c->drop_batch_cache();
         c->graphs.erase(&g);
         free_mirror(kv.second);
     }
     g.dev.clear();
 }
 
-Status ensure_arena(Ctx &ctx, uint64_t bytes)
+Status ensure_arena(Arena &a, uint64_t bytes)
 {
-    if (ctx.arena.bytes >= bytes) return Status::ok();
-    if (ctx.arena.base) {
-        (void)hipFree(ctx.arena.base);
-        ctx.arena = Arena{};
+    if (a.bytes >= bytes) return Status::ok();
+    if (a.base) {
+        (void)hipFree(a.base);
+        a = Arena{};
     }
-    hipError_t e = hipMalloc(reinterpret_cast<void **>(&ctx.arena.base), bytes);
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(&a.base), bytes);
     if (e != hipSuccess) {
-        ctx.arena = Arena{};
+        a = Arena{};
         return Status::err(GFSM_ERR_NOMEM, "cannot allocate " + std::to_string(bytes) + " device bytes");
     }
-    ctx.arena.bytes = bytes;
+    a.bytes = bytes;
     return Status::ok();
 }
+
+Status ensure_arena(Ctx &ctx, uint64_t bytes) { return ensure_arena(ctx.arena, bytes); }
 
 template <class T>
 static Status dev_copy(T *&dst, const std::vector<T> &src, size_t &bytes)
@@ -310,14 +315,10 @@ struct Runner {
         return (kind == EXPAND_MID && plan.dev.depth[d].nxt_dist > 1) ? 1u : 0u;
     }
 
-    // Candidates per thread for one expand launch: the tuned (or forced)
-    // value, reduced while the LDS staging would exceed 64 KB (two
-    // workgroups per CU).
+    // Candidates per thread for one expand launch: forced, else tuned.
     uint32_t pick_items(uint32_t kind, uint32_t d) const
     {
-        uint32_t it = forced_items ? forced_items : ctx.tune.items[kind][d];
-        while (it > 1 && expand_lds_bytes(plan.dev.depth[d].nchk, need_nxt(kind, d), it) > 64 * 1024) it >>= 1;
-        return it;
+        return forced_items ? forced_items : ctx.tune.items[kind][d];
     }
 
     // Candidates per launch: the workspace holds max_tiles tiles, and the
@@ -343,7 +344,7 @@ struct Runner {
         rows_mode = flags & (GFSM_ROWS | GFSM_SORTED);
         digest = (flags & GFSM_DIGEST) && !rows_mode;  // rows mode: digest from the host rows
         prof = flags & GFSM_PROFILE;
-        grid_cap = std::max<uint32_t>(ctx.num_cus, 1) * 8;
+        grid_cap = std::max<uint32_t>(ctx.num_cus, 1) * ctx.tune.wgs_per_cu;
         const uint64_t min_nodes = 8 * kTile;  // a level must hold one launch tile's survivors
         uint64_t row_bytes = 0;
         if (rows_mode) {
@@ -416,24 +417,24 @@ struct Runner {
             uint64_t e = std::min(rhi, c + step);
             RootArgs a{};
             a.g = dg.view;
-            a.cand_vtx = dg.rel_vtx + c;
-            a.cand_beg = dg.rel_beg + c;
-            a.cand_cnt = dg.rel_cnt + c;
-            a.n = e - c;
+            a.io.cand_vtx = dg.rel_vtx + c;
+            a.io.cand_beg = dg.rel_beg + c;
+            a.io.cand_cnt = dg.rel_cnt + c;
+            a.io.n = e - c;
+            a.io.out_vec = vec[0];
+            a.io.out_off = off[0];
+            a.io.out_gbeg = gbeg[0];
+            a.io.cap = cap[0];
+            a.io.tot = ctx.lane.tot_d;
+            a.io.ntiles = (a.io.n + kRootTile - 1) / kRootTile;
             a.nchk = plan.dev.nroot_chk;
             std::memcpy(a.chk_rel, plan.dev.root_chk_rel, sizeof(a.chk_rel));
-            a.out_vec = vec[0];
-            a.out_off = off[0];
-            a.out_gbeg = gbeg[0];
-            a.cap = cap[0];
             a.ws = ctx.lane.ws;
-            a.tot = ctx.lane.tot_d;
-            a.ntiles = (a.n + kRootTile - 1) / kRootTile;
-            if (a.ntiles > ctx.max_tiles || e > data.rel_vtx.size())
+            if (a.io.ntiles > ctx.max_tiles || e > data.rel_vtx.size())
                 return Status::err(GFSM_ERR_ARG, "internal: root launch shape check failed");
             HIP_TRY(ctx.lane.before_launch());
             kbegin();
-            HIP_TRY(launch_root(a, grid_for(a.ntiles), ctx.lane.stream));
+            HIP_TRY(launch_root(a, grid_for(a.io.ntiles), ctx.lane.stream));
             res.stats.launches++;
             Status st = kend(false);
             if (!st.good()) return st;
@@ -456,51 +457,53 @@ struct Runner {
         const uint32_t kind = last ? (rows_mode ? EXPAND_LAST_ROWS : EXPAND_LAST_COUNT) : EXPAND_MID;
         const uint64_t tile_limit = launch_limit();
         const uint32_t items = pick_items(kind, d);
+        const uint32_t pool_words = ctx.tune.pool_words;
         const uint64_t tc = (uint64_t)kTile * items;
         bool first = true;
         for (uint64_t c = g0; c < g1;) {
             const uint64_t e = std::min(g1, c + std::max<uint64_t>(tc, std::min(chunk[d], tile_limit)));
             ExpandArgs a{};
             a.items = items;
-            a.need_nxt = need_nxt(kind, d);
+            a.pool_words = pool_words;
             a.g = dg.view;
             a.dp = plan.dev.depth[d];
-            a.d = d;
-            a.q = q;
-            a.kind = kind;
-            a.digest = (last && digest) ? 1u : 0u;
             std::memcpy(a.col, plan.dev.col, sizeof(a.col));
             std::memcpy(a.dcol, plan.dev.dcol, sizeof(a.dcol));
-            a.in_vec = vec[d - 1];
-            a.in_off = off[d - 1];
-            a.in_gbeg = gbeg[d - 1];
-            a.S_prev = S_prev;
-            a.g_begin = c;
-            a.g_end = e;
+            StepIO &io = a.io;
+            io.need_nxt = need_nxt(kind, d);
+            io.d = d;
+            io.q = q;
+            io.kind = kind;
+            io.digest = (last && digest) ? 1u : 0u;
+            io.in_vec = vec[d - 1];
+            io.in_off = off[d - 1];
+            io.in_gbeg = gbeg[d - 1];
+            io.S_prev = S_prev;
+            io.g_begin = c;
+            io.g_end = e;
             if (!last) {
-                a.out_vec = vec[d];
-                a.out_off = off[d];
-                a.out_gbeg = gbeg[d];
-                a.cap = cap[d];
+                io.out_vec = vec[d];
+                io.out_off = off[d];
+                io.out_gbeg = gbeg[d];
+                io.cap = cap[d];
             } else if (rows_mode) {
-                a.rows = rows_d;
-                a.cap = rows_cap;
+                io.rows = rows_d;
+                io.cap = rows_cap;
             }
+            io.tot = ctx.lane.tot_d;
+            io.ntiles = (e - c + tc - 1) / tc;
             a.ws = ctx.lane.ws;
-            a.tot = ctx.lane.tot_d;
-            a.ntiles = (e - c + tc - 1) / tc;
             // Launch-shape checks: the workspace holds every tile, the frontier
             // is non-empty and the LDS staging fits.
-            if (a.ntiles == 0 || a.ntiles > ctx.max_tiles || S_prev == 0 || S_prev > cap[d - 1] ||
-                a.dp.gen_dist == 0 || a.dp.gen_dist > d ||
-                expand_lds_bytes(a.dp.nchk, a.need_nxt, items) > 150 * 1024)
+            if (io.ntiles == 0 || io.ntiles > ctx.max_tiles || S_prev == 0 || S_prev > cap[d - 1] ||
+                a.dp.gen_dist == 0 || a.dp.gen_dist > d || expand_lds_bytes(items, pool_words) > kMaxDynLds)
                 return Status::err(GFSM_ERR_ARG, "internal: expand launch shape check failed");
             if (max_cand && cand_total + (e - c) > max_cand)
                 return Status::err(GFSM_ERR_CAP, "candidate cap reached");
             cand_total += e - c;
             HIP_TRY(ctx.lane.before_launch());
             kbegin();
-            HIP_TRY(launch_expand(a, grid_for(a.ntiles), ctx.lane.stream));
+            HIP_TRY(launch_expand(a, grid_for(io.ntiles), ctx.lane.stream));
             res.stats.launches++;
             if (!first) res.stats.chunks++;
             first = false;

@@ -46,10 +46,19 @@ struct Arena {
     uint64_t bytes = 0;
 };
 
-// Tile shapes of the expand kernel: candidates per thread (1, 2, 4, 8) per
-// kernel kind and depth.  Filled with defaults, replaced by the autotuner.
+// Tile shapes: candidates per thread (1, 2, 4, 8) per kernel kind and depth
+// for the single-query path, the LDS staging pool, workgroups per CU of the
+// persistent grids, and the batch step kernel's shape.  Filled with
+// defaults, replaced by the autotuner.
 struct Tuning {
     uint8_t items[3][GFSM_MAXQ];
+    uint32_t pool_words = kPoolWords;
+    uint32_t wgs_per_cu = 8;
+    uint32_t batch_items = 4;
+    uint32_t batch_pool_words = kPoolWords;
+    uint32_t batch_wgs_per_cu = 8;
+    uint32_t batch_slots = 256;
+    uint32_t batch_steps = 8;  // steps per captured replay
     Tuning();
 };
 
@@ -60,14 +69,27 @@ struct Ctx {
     uint64_t max_tiles = 0;
     Lane lane;
     Arena arena;
+    Arena batch_arena;  // level regions of the batched path (grow-only, shared by its engines)
     Tuning tune;
     std::mutex mu;
     std::set<const HostGraph *> graphs;  // data graphs resident on this context
+    // Batched-matching engine kept between calls (its arena, slots and
+    // captured steps); owned by batch.cpp, dropped with its graph.
+    void *batch_cache = nullptr;
+    const HostGraph *batch_graph = nullptr;
+    void (*batch_cache_free)(void *) = nullptr;
+    void drop_batch_cache()
+    {
+        if (batch_cache && batch_cache_free) batch_cache_free(batch_cache);
+        batch_cache = nullptr;
+        batch_graph = nullptr;
+    }
 };
 
 Status ctx_open(int device, uint64_t budget, Ctx *&out);
 void ctx_close(Ctx *ctx);
 Status ensure_arena(Ctx &ctx, uint64_t bytes);
+Status ensure_arena(Arena &a, uint64_t bytes);
 Status upload_graph(Ctx &ctx, const HostGraph &g, DevGraph *&out);
 void release_graph_mirrors(const HostGraph &g);
 

@@ -35,7 +35,7 @@ struct ScanWS {
     uint64_t max_tiles;
 };
 
-// Results of a launch, written on the device, read by the host afterwards.
+// Results of one step, written on the device, read by the host afterwards.
 struct Totals {
     uint64_t keep;    // nodes kept (last level: embeddings when rows are written)
     uint64_t cnt;     // candidates of the next depth
@@ -50,65 +50,169 @@ enum ExpandKind : uint32_t {
     EXPAND_LAST_ROWS = 2,  // last depth: write rows in original ids (and digest)
 };
 
+// Data movement of one expand step over a candidate range of depth d.
 // A frontier level holding nodes with k+1 placed vertices (depths 0..k):
 //   vec[i*(k+1) + j] = data vertex placed at depth j by node i,
 //   off[i] = exclusive prefix of the nodes' next-depth candidate counts
 //            (off[S] = total), strictly increasing (every node has >= 1),
 //   gbeg[i] = begin (in nbr) of the node's next-depth generator segment.
 // The next depth's candidate g of node i is nbr[gbeg[i] + (g - off[i])].
-struct ExpandArgs {
-    DevGraphView g;
-    DepthPlanDev dp;
-    uint32_t d, q;
-    uint32_t kind;
-    uint32_t digest;
-    uint32_t col[GFSM_MAXQ];   // original query id of depth k
-    uint32_t dcol[GFSM_MAXQ];  // depth of column c
+struct StepIO {
     const uint32_t *in_vec;    // level d-1 (width d)
     const uint64_t *in_off;    // [S_prev+1]
     const uint32_t *in_gbeg;   // [S_prev]
     uint64_t S_prev;
-    uint64_t g_begin, g_end;   // candidate range of this launch
+    uint64_t g_begin, g_end;   // candidate range of this step
     uint32_t *out_vec;         // level d (width d+1), EXPAND_MID
     uint64_t *out_off;         // [cap+1]
     uint32_t *out_gbeg;        // [cap]
     uint32_t *rows;            // [cap*q], EXPAND_LAST_ROWS
     uint64_t cap;
-    ScanWS ws;
     Totals *tot;
     uint64_t ntiles;
-    uint32_t items;            // candidates per thread: 1, 2, 4 or 8 (tile = 256 * items)
+    uint32_t d, q;
+    uint32_t kind;
+    uint32_t digest;
     uint32_t need_nxt;         // stage the next generator per node (MID, generator not the new vertex)
+    uint32_t pad;
 };
 
-struct RootArgs {
-    DevGraphView g;
-    const uint32_t *cand_vtx;  // root candidates (a slice of the relation index)
+// Root step: the candidates of depth 0 (a slice of the relation index).
+struct RootIO {
+    const uint32_t *cand_vtx;  // root candidates
     const uint32_t *cand_beg;  // their depth-1 generator segment begin
     const uint32_t *cand_cnt;  // ... and length
     uint64_t n;
-    uint32_t nchk;             // self-loop relations of the root
-    uint32_t chk_rel[GFSM_MAXCHK];
     uint32_t *out_vec;         // level 0 (width 1)
     uint64_t *out_off;
     uint32_t *out_gbeg;
     uint64_t cap;
-    ScanWS ws;
     Totals *tot;
     uint64_t ntiles;
 };
 
+struct ExpandArgs {
+    DevGraphView g;
+    DepthPlanDev dp;
+    uint32_t col[GFSM_MAXQ];   // original query id of depth k
+    uint32_t dcol[GFSM_MAXQ];  // depth of column c
+    StepIO io;
+    ScanWS ws;
+    uint32_t items;            // candidates per thread: 1, 2, 4 or 8 (tile = 256 * items)
+    uint32_t pool_words;       // LDS words for per-node staging (checked segments, rows)
+};
+
+struct RootArgs {
+    DevGraphView g;
+    RootIO io;
+    uint32_t nchk;             // self-loop relations of the root
+    uint32_t chk_rel[GFSM_MAXCHK];
+    ScanWS ws;
+};
+
 constexpr uint32_t kTile = 256;     // threads per workgroup
 constexpr uint32_t kItems = 8;      // default candidates per thread per tile
-constexpr uint32_t kRootItems = 4;  // root level
+constexpr uint32_t kRootItems = 4;  // root level (single-query path)
 constexpr uint32_t kRootTile = kTile * kRootItems;
+constexpr uint32_t kPoolWords = 8192;  // default staging pool (32 KB)
 
-// LDS bytes the expand kernel needs with `nchk` ancestor checks, per-node
-// next-generator staging (`need_nxt`) and `items` candidates per thread.
-size_t expand_lds_bytes(uint32_t nchk, uint32_t need_nxt, uint32_t items);
+// Dynamic LDS bytes of the expand and batch-step kernels for `items`
+// candidates per thread and a staging pool of `pool_words` words.
+size_t expand_lds_bytes(uint32_t items, uint32_t pool_words);
+constexpr size_t kMaxDynLds = 150 * 1024;
 
 hipError_t launch_root(const RootArgs &a, uint32_t grid, hipStream_t s);
 hipError_t launch_expand(const ExpandArgs &a, uint32_t grid, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Batched matching: many queries share every step.  Each query owns a slot in
+// device memory; a scheduler kernel advances every slot by one step of its
+// own depth-first chunk walk and lays the slots' tiles end to end; one
+// persistent step kernel then works through all tiles of all slots, handed
+// out by ticket (a device work queue of (slot, tile) items).  A slot's chained
+// scan spans only its own tiles.  No host round trip is needed between
+// steps, so runs of steps are captured in a hipGraph and replayed.
+enum SlotState : uint32_t {
+    SLOT_FREE = 0,      // unused
+    SLOT_NEW = 1,       // admitted by the host; the root step is scheduled next
+    SLOT_RUN = 2,       // running
+    SLOT_DONE = 3,      // finished: count and digest are final
+    SLOT_CAP = 4,       // the work cap was reached: partial count
+    SLOT_ERROR = 5,     // a step shape check failed (see error)
+};
+constexpr uint32_t STEP_NONE = 7;
+constexpr uint32_t STEP_ROOT = 8;
+constexpr uint32_t kMaxSlots = 1024;
+
+// Per-slot summary the host polls (one small copy for all slots).
+struct SlotReport {
+    uint32_t state;
+    uint32_t steps;
+    uint64_t count, dg0, dg1;
+};
+
+struct BatchSlot {
+    // Written by the host at admission, read-only on the device.
+    const QueryPlanDev *plan;     // device copy of the query plan
+    const uint32_t *root_vtx, *root_beg, *root_cnt;  // root candidates
+    uint64_t nroots;
+    uint32_t *vec[GFSM_MAXQ];     // level buffers (the slot's arena region)
+    uint64_t *off[GFSM_MAXQ];
+    uint32_t *gbeg[GFSM_MAXQ];
+    uint64_t cap[GFSM_MAXQ];
+    uint64_t max_cand;            // work cap (0 = none)
+    uint32_t q, digest;
+    uint32_t state;               // SLOT_NEW at admission
+    uint32_t error;
+    // Depth-first chunk walk, maintained by the scheduler.
+    uint32_t step;                // STEP_NONE, STEP_ROOT or an ExpandKind
+    uint32_t d;                   // depth of the current step
+    uint64_t root_next;           // next root candidate
+    uint64_t lo[GFSM_MAXQ], hi[GFSM_MAXQ];  // remaining candidate range per depth
+    uint64_t S[GFSM_MAXQ];        // nodes in level k
+    uint64_t chunk[GFSM_MAXQ];    // candidates per step per depth (chunk[0]: roots)
+    uint64_t cur_lo, cur_hi;      // range of the scheduled step
+    uint64_t tile0;               // global index of its first tile
+    uint64_t cand_done;
+    uint64_t count, dg0, dg1;     // result (embeddings, digest sums)
+    uint64_t level[GFSM_MAXQ], cand[GFSM_MAXQ];  // statistics
+    uint32_t steps, overflows;
+    Totals tot;                   // totals of the scheduled step
+    // Arguments of the scheduled step (written by the scheduler).
+    StepIO io;
+    RootIO rio;
+};
+
+struct BatchCtl {
+    uint64_t total_tiles;         // tiles of the scheduled step, all slots
+    uint64_t step_cand;           // candidates of one slot in one step (set by the host
+                                  // between replays; 0 = max_step_cand)
+    uint32_t running;             // slots in SLOT_NEW or SLOT_RUN after scheduling
+    uint32_t pad;
+};
+
+// Batch configuration shared by the scheduler and the step kernel.
+struct BatchCfg {
+    BatchSlot *slots;
+    SlotReport *report;           // [nslots]
+    uint64_t *tile_start;         // [nslots + 1]
+    BatchCtl *ctl;
+    uint32_t nslots;
+    uint32_t items;               // candidates per thread (tile = 256 * items)
+    uint32_t pool_words;
+    uint32_t pad;
+    uint64_t max_step_cand;       // upper bound of BatchCtl::step_cand (workspace size)
+    uint64_t max_degree;          // of the data graph (scan value bound)
+    DevGraphView g;
+    ScanWS ws;
+};
+
+// One batch step: the scheduler (one workgroup) consumes every slot's last
+// step, schedules its next one and lays out the tiles; the step kernel
+// (`grid` persistent workgroups) runs them.  Both read only `cfg` and device
+// memory, so the pair can be captured and replayed.
+hipError_t launch_batch_schedule(const BatchCfg &cfg, hipStream_t s);
+hipError_t launch_batch_step(const BatchCfg &cfg, uint32_t grid, hipStream_t s);
 
 // Chained-scan self test: out[i] = exclusive prefix of vals (u64), and the
 // indices i with vals[i] != 0 compacted in order into idx; totals in tot.

@@ -1,22 +1,29 @@
 // kernels.hip -- CDNA4 (gfx950) kernels of the forest-based matcher.
 //
-// One launch per (depth, candidate range).  A workgroup of 256 threads
-// (4 wave64s) owns a tile of 256 * I consecutive candidates of the depth
-// (I = 1, 2, 4 or 8 per thread, striped so that loads stay coalesced):
+// A step expands one candidate range of one depth.  A workgroup of 256
+// threads (4 wave64s) owns a tile of 256 * I consecutive candidates of the
+// step (I = 1, 2, 4 or 8 per thread, striped so that loads stay coalesced):
 //   1. one wave finds the frontier nodes whose candidate ranges cover the
 //      tile (a 32-ary search per tile end over their exclusive offsets) and
-//      marks where each node's candidates start in an LDS bitmap,
+//      marks where each node's candidates start in an LDS bitmap; the nodes'
+//      placed vertices, checked segments and next generators are staged in an
+//      LDS pool when they fit,
 //   2. each candidate finds its node from the bitmap (mbcnt of a wave-uniform
 //      word), reads itself coalesced from the node's generator segment, is
-//      checked for injectivity against the node's placed vertices (a row of
-//      the previous level, read through the cache) and binary-searched in the
-//      segments of the checked ancestors (staged per node in LDS),
+//      checked for injectivity against the node's placed vertices and
+//      searched in the segments of the checked ancestors -- the I searches of
+//      a thread advance in lockstep, so each thread keeps I loads in flight,
 //   3. survivors are compacted with wave64 ballot/mbcnt counts and DPP prefix
 //      sums of their next-depth candidate counts, chained across workgroups
 //      by a decoupled look-back, so that the next level -- rows, candidate
 //      offsets and generator segment begins -- comes out of the same pass.
 // This replaces hnGen + hnCheck + CUB ExclusiveSum + hnRemove + the degree
 // scan of one reference depth (HnGPU.cu:114-262, 309-349, 371-439).
+//
+// The single-query kernels (k_root, k_expand) run one step per launch.  The
+// batch kernels run one step of every admitted query per launch: a scheduler
+// kernel advances each query's depth-first chunk walk on the device, and a
+// persistent step kernel takes (query, tile) work items by ticket.
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
@@ -68,45 +75,96 @@ DEV uint64_t mix64(uint64_t z)
 }
 
 // ---------------------------------------------------------------------------
-// graph access
+// graph access, I lookups per thread in lockstep
 // ---------------------------------------------------------------------------
 
-// [beg, end) of the (x, rel) segment, {0, 0} when x has none.
-DEV uint2 seg_lookup(const DevGraphView &g, uint32_t x, uint32_t rel)
+// seg[k] = [beg, end) of the (x[k], rel) segment, {0, 0} when x[k] has none
+// or act[k] is false.  Finds the last directory entry of x[k] whose relation
+// is <= rel with a branch-free halving search; all I searches issue their
+// loads together.
+template <uint32_t I>
+DEV void seg_lookup_n(const DevGraphView &g, const uint32_t (&x)[I], uint32_t rel, const bool (&act)[I],
+                      uint2 (&seg)[I])
 {
     const uint2 *dir = reinterpret_cast<const uint2 *>(g.dir);
-    uint32_t lo = g.dir_ptr[x], hi = g.dir_ptr[x + 1];
-    uint32_t n = hi - lo;
-    while (n > 0) {
-        uint32_t h = n >> 1;
-        if (dir[lo + h].x < rel) {
-            lo += h + 1;
-            n -= h + 1;
-        } else {
-            n = h;
+    uint32_t lo[I], n[I];
+#pragma unroll
+    for (uint32_t k = 0; k < I; k++) {
+        lo[k] = 0;
+        n[k] = 0;
+        if (act[k]) {
+            lo[k] = g.dir_ptr[x[k]];
+            n[k] = g.dir_ptr[x[k] + 1];
         }
     }
-    if (lo < hi) {
-        uint2 e = dir[lo];
-        if (e.x == rel) return make_uint2(e.y, dir[lo + 1].y);
+#pragma unroll
+    for (uint32_t k = 0; k < I; k++) n[k] -= lo[k];
+    for (;;) {
+        bool more = false;
+#pragma unroll
+        for (uint32_t k = 0; k < I; k++) more |= n[k] > 1;
+        if (!more) break;
+        uint32_t key[I];
+#pragma unroll
+        for (uint32_t k = 0; k < I; k++) key[k] = (n[k] > 1) ? dir[lo[k] + (n[k] >> 1)].x : 0u;
+#pragma unroll
+        for (uint32_t k = 0; k < I; k++) {
+            if (n[k] > 1) {
+                const uint32_t h = n[k] >> 1;
+                if (key[k] <= rel) lo[k] += h;
+                n[k] -= h;
+            }
+        }
     }
-    return make_uint2(0u, 0u);
+    uint2 e[I];
+    uint32_t nb[I];
+#pragma unroll
+    for (uint32_t k = 0; k < I; k++) {
+        e[k] = make_uint2(~0u, 0u);
+        nb[k] = 0;
+        if (n[k]) {
+            e[k] = dir[lo[k]];
+            nb[k] = dir[lo[k] + 1].y;  // the directory ends with a sentinel entry
+        }
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < I; k++) seg[k] = (e[k].x == rel) ? make_uint2(e[k].y, nb[k]) : make_uint2(0u, 0u);
 }
 
-// Is w in the sorted slice nbr[beg, end)?
-DEV bool seg_contains(const uint32_t *__restrict__ nbr, uint32_t beg, uint32_t end, uint32_t w)
+// keep[k] &= (w[k] is in the sorted slice nbr[seg[k].x, seg[k].y)): the last
+// element <= w[k] by halving, then one compare; I searches in lockstep.
+template <uint32_t I>
+DEV void seg_contains_n(const uint32_t *__restrict__ nbr, const uint2 (&seg)[I], const uint32_t (&w)[I],
+                        bool (&keep)[I])
 {
-    uint32_t lo = beg, n = end - beg;
-    while (n > 0) {
-        uint32_t h = n >> 1;
-        if (nbr[lo + h] < w) {
-            lo += h + 1;
-            n -= h + 1;
-        } else {
-            n = h;
+    uint32_t lo[I], n[I];
+#pragma unroll
+    for (uint32_t k = 0; k < I; k++) {
+        lo[k] = seg[k].x;
+        n[k] = keep[k] ? seg[k].y - seg[k].x : 0u;
+    }
+    for (;;) {
+        bool more = false;
+#pragma unroll
+        for (uint32_t k = 0; k < I; k++) more |= n[k] > 1;
+        if (!more) break;
+        uint32_t v[I];
+#pragma unroll
+        for (uint32_t k = 0; k < I; k++) v[k] = (n[k] > 1) ? nbr[lo[k] + (n[k] >> 1)] : 0u;
+#pragma unroll
+        for (uint32_t k = 0; k < I; k++) {
+            if (n[k] > 1) {
+                const uint32_t h = n[k] >> 1;
+                if (v[k] <= w[k]) lo[k] += h;
+                n[k] -= h;
+            }
         }
     }
-    return lo < end && nbr[lo] == w;
+    uint32_t v[I];
+#pragma unroll
+    for (uint32_t k = 0; k < I; k++) v[k] = n[k] ? nbr[lo[k]] : 0u;
+#pragma unroll
+    for (uint32_t k = 0; k < I; k++) keep[k] = (n[k] != 0) && v[k] == w[k];
 }
 
 // ---------------------------------------------------------------------------
@@ -120,10 +178,11 @@ DEV uint64_t pack(uint64_t st, uint32_t ep, uint64_t v)
     return (st << 62) | ((uint64_t)(ep & ((1u << kEpochBits) - 1)) << 46) | (v & kValueMask);
 }
 
-// Exclusive prefix of tile t over one word array, computed by all 64 lanes of
+// Exclusive prefix of tile t over one word array, where the scan starts at
+// tile t0 (the first tile of the step's query), computed by all 64 lanes of
 // one wave: each pass reads the words of the 64 preceding tiles, waits until
 // every tile up to the nearest inclusive prefix has published, and sums.
-DEV uint64_t lookback(const uint64_t *words, uint32_t ep, uint64_t t)
+DEV uint64_t lookback(const uint64_t *words, uint32_t ep, uint64_t t, uint64_t t0)
 {
     const uint32_t lane = lane_id();
     const uint64_t epm = (uint64_t)(ep & ((1u << kEpochBits) - 1));
@@ -134,9 +193,9 @@ DEV uint64_t lookback(const uint64_t *words, uint32_t ep, uint64_t t)
         uint64_t w = 0, pm;
         uint32_t st;
         for (;;) {
-            st = 2u;  // before tile 0 the inclusive prefix is 0
+            st = 2u;  // before the first tile the inclusive prefix is 0
             w = 0;
-            if (idx >= 0) {
+            if (idx >= (int64_t)t0) {
                 w = ld_word(words + idx);
                 st = (((w >> 46) & ((1u << kEpochBits) - 1)) == epm) ? (uint32_t)(w >> 62) : 0u;
             }
@@ -159,16 +218,55 @@ template <uint32_t I>
 struct ScanSmem {
     uint32_t wk[I][kWaves];
     uint64_t wc[I][kWaves];
-    uint64_t ex[2];
+    uint64_t ex[2];   // the tile's exclusive prefix (kept, candidates)
+    uint64_t tot[2];  // ... and inclusive
 };
 
-// Tile-wide exclusive prefix of (keep, cnt) in candidate order (item k of
-// thread x is tile position k * 256 + x), continued across tiles.  Returns
-// each item's kept index and candidate offset and the totals through this
-// tile.
+// Publish this tile's totals (the per-(item, wave) counts already in sm) and
+// learn its exclusive prefix: wave 0 chains with the preceding tiles of the
+// scan that starts at tile t0.
 template <uint32_t I>
-DEV void tile_scan(const bool (&keep)[I], const uint32_t (&cnt)[I], uint64_t t, const ScanWS &ws, uint32_t ep,
-                   ScanSmem<I> &sm, uint64_t (&my_k)[I], uint64_t (&my_c)[I], uint64_t &run_k, uint64_t &run_c)
+DEV void tile_prefix(uint64_t t, uint64_t t0, const ScanWS &ws, uint32_t ep, ScanSmem<I> &sm)
+{
+    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+    if (wave == 0) {
+        uint64_t ak = 0, ac = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < I; k++)
+#pragma unroll
+            for (uint32_t w = 0; w < kWaves; w++) {
+                ak += sm.wk[k][w];
+                ac += sm.wc[k][w];
+            }
+        uint64_t ek = 0, ec = 0;
+        if (t != t0) {
+            if (lane == 0) {
+                st_word(ws.wk + t, pack(1, ep, ak));
+                st_word(ws.wc + t, pack(1, ep, ac));
+            }
+            ek = lookback(ws.wk, ep, t, t0);
+            ec = lookback(ws.wc, ep, t, t0);
+        }
+        if (lane == 0) {
+            st_word(ws.wk + t, pack(2, ep, ek + ak));
+            st_word(ws.wc + t, pack(2, ep, ec + ac));
+            sm.ex[0] = ek;
+            sm.ex[1] = ec;
+            sm.tot[0] = ek + ak;
+            sm.tot[1] = ec + ac;
+        }
+    }
+    __syncthreads();
+}
+
+// Tile-wide exclusive prefix of (keep, cnt) in candidate order (item k of
+// thread x is tile position k * 256 + x), continued across the tiles of one
+// scan.  Returns each item's kept index and candidate offset and the totals
+// through this tile.
+template <uint32_t I>
+DEV void tile_scan(const bool (&keep)[I], const uint32_t (&cnt)[I], uint64_t t, uint64_t t0, const ScanWS &ws,
+                   uint32_t ep, ScanSmem<I> &sm, uint64_t (&my_k)[I], uint64_t (&my_c)[I], uint64_t &run_k,
+                   uint64_t &run_c)
 {
     const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
     uint32_t kpre[I], cpre[I];
@@ -184,44 +282,25 @@ DEV void tile_scan(const bool (&keep)[I], const uint32_t (&cnt)[I], uint64_t t, 
         }
     }
     __syncthreads();
-    uint64_t ak = 0, ac = 0, offk[I], offc[I];
+    tile_prefix<I>(t, t0, ws, ep, sm);
+    uint64_t bk = sm.ex[0], bc = sm.ex[1];
 #pragma unroll
     for (uint32_t k = 0; k < I; k++) {
+        uint64_t ok_ = bk, oc_ = bc;
 #pragma unroll
         for (uint32_t w = 0; w < kWaves; w++) {
-            if (w == wave) {
-                offk[k] = ak;
-                offc[k] = ac;
+            if (w < wave) {
+                ok_ += sm.wk[k][w];
+                oc_ += sm.wc[k][w];
             }
-            ak += sm.wk[k][w];
-            ac += sm.wc[k][w];
+            bk += sm.wk[k][w];
+            bc += sm.wc[k][w];
         }
+        my_k[k] = ok_ + kpre[k];
+        my_c[k] = oc_ + cpre[k];
     }
-    if (wave == 0) {
-        uint64_t ek = 0, ec = 0;
-        if (t != 0) {
-            if (lane == 0) {
-                st_word(ws.wk + t, pack(1, ep, ak));
-                st_word(ws.wc + t, pack(1, ep, ac));
-            }
-            ek = lookback(ws.wk, ep, t);
-            ec = lookback(ws.wc, ep, t);
-        }
-        if (lane == 0) {
-            st_word(ws.wk + t, pack(2, ep, ek + ak));
-            st_word(ws.wc + t, pack(2, ep, ec + ac));
-            sm.ex[0] = ek;
-            sm.ex[1] = ec;
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (uint32_t k = 0; k < I; k++) {
-        my_k[k] = sm.ex[0] + offk[k] + kpre[k];
-        my_c[k] = sm.ex[1] + offc[k] + cpre[k];
-    }
-    run_k = sm.ex[0] + ak;
-    run_c = sm.ex[1] + ac;
+    run_k = sm.tot[0];
+    run_c = sm.tot[1];
 }
 
 DEV uint32_t read_epoch(const ScanWS &ws)
@@ -262,8 +341,8 @@ DEV void finish_launch(const ScanWS &ws)
     }
 }
 
-// Sum (count, digest0, digest1) over the workgroup and add it to the launch
-// totals: one set of atomics per workgroup, not per wave.
+// Sum (count, digest0, digest1) over the workgroup and add it to `tot`: one
+// set of atomics per workgroup, not per wave.
 DEV void flush_sums(uint64_t cnt, uint64_t h0, uint64_t h1, Totals *tot, uint64_t (*s_red)[kWaves])
 {
     const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
@@ -289,57 +368,77 @@ DEV void flush_sums(uint64_t cnt, uint64_t h0, uint64_t h1, Totals *tot, uint64_
     }
 }
 
+// Statically allocated LDS of one tile loop.
+template <uint32_t I>
+struct TileShared {
+    ScanSmem<I> sm;
+    uint64_t tile;
+    uint64_t node[2];
+    uint64_t red[3][kWaves];
+    uint32_t slot;
+};
+
 // ---------------------------------------------------------------------------
 // root level (hnInit, HnGPU.cu:76-109)
 // ---------------------------------------------------------------------------
 
+// Root tile `lt` of a root step (global tile gt, the step's scan starting at
+// gt0): keep candidates that carry every self-loop relation of the root.
+template <uint32_t I>
+DEV void root_tile(const DevGraphView &g, const RootIO &io, uint32_t nchk, const uint32_t *chk_rel, uint64_t lt,
+                   uint64_t gt, uint64_t gt0, const ScanWS &ws, uint32_t ep, ScanSmem<I> &sm)
+{
+    bool keep[I];
+    uint32_t v[I], cnt[I];
+#pragma unroll
+    for (uint32_t k = 0; k < I; k++) {
+        const uint64_t c = lt * (kTile * I) + k * kTile + threadIdx.x;
+        keep[k] = false;
+        v[k] = 0;
+        cnt[k] = 0;
+        if (c < io.n) {
+            v[k] = io.cand_vtx[c];
+            cnt[k] = io.cand_cnt[c];
+            keep[k] = cnt[k] > 0;
+        }
+    }
+    for (uint32_t r = 0; r < nchk; r++) {
+        uint2 sg[I];
+        seg_lookup_n<I>(g, v, chk_rel[r], keep, sg);
+        seg_contains_n<I>(g.nbr, sg, v, keep);
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < I; k++)
+        if (!keep[k]) cnt[k] = 0;
+    uint64_t kk[I], off[I], rk, rc;
+    tile_scan<I>(keep, cnt, gt, gt0, ws, ep, sm, kk, off, rk, rc);
+#pragma unroll
+    for (uint32_t k = 0; k < I; k++) {
+        if (keep[k] && kk[k] < io.cap) {
+            const uint64_t c = lt * (kTile * I) + k * kTile + threadIdx.x;
+            io.out_vec[kk[k]] = v[k];
+            io.out_off[kk[k]] = off[k];
+            io.out_gbeg[kk[k]] = io.cand_beg[c];
+        }
+    }
+    if (lt == io.ntiles - 1 && threadIdx.x == 0) {
+        io.tot->keep = rk;
+        io.tot->cnt = rc;
+        if (rk <= io.cap) io.out_off[rk] = rc;
+    }
+}
+
 template <uint32_t I>
 __global__ __launch_bounds__(kTile) void k_root(RootArgs a)
 {
-    __shared__ ScanSmem<I> sm;
-    __shared__ uint64_t s_tile;
+    __shared__ TileShared<I> sh;
     const uint32_t ep = read_epoch(a.ws);
     uint32_t next = 0;
     tickets_init(a.ws, next);
     for (;;) {
-        const uint64_t t = tickets_take(a.ws, next, &s_tile, a.ntiles);
-        if (t >= a.ntiles) break;
-        bool keep[I];
-        uint32_t v[I], cnt[I];
-#pragma unroll
-        for (uint32_t k = 0; k < I; k++) {
-            const uint64_t c = t * (kTile * I) + k * kTile + threadIdx.x;
-            keep[k] = false;
-            v[k] = 0;
-            cnt[k] = 0;
-            if (c < a.n) {
-                v[k] = a.cand_vtx[c];
-                cnt[k] = a.cand_cnt[c];
-                bool ok = cnt[k] > 0;
-                for (uint32_t r = 0; r < a.nchk && ok; r++) {
-                    uint2 sg = seg_lookup(a.g, v[k], a.chk_rel[r]);
-                    ok = seg_contains(a.g.nbr, sg.x, sg.y, v[k]);
-                }
-                keep[k] = ok;
-                if (!ok) cnt[k] = 0;
-            }
-        }
-        uint64_t kk[I], off[I], rk, rc;
-        tile_scan<I>(keep, cnt, t, a.ws, ep, sm, kk, off, rk, rc);
-#pragma unroll
-        for (uint32_t k = 0; k < I; k++) {
-            if (keep[k] && kk[k] < a.cap) {
-                const uint64_t c = t * (kTile * I) + k * kTile + threadIdx.x;
-                a.out_vec[kk[k]] = v[k];
-                a.out_off[kk[k]] = off[k];
-                a.out_gbeg[kk[k]] = a.cand_beg[c];
-            }
-        }
-        if (t == a.ntiles - 1 && threadIdx.x == 0) {
-            a.tot->keep = rk;
-            a.tot->cnt = rc;
-            if (rk <= a.cap) a.out_off[rk] = rc;
-        }
+        const uint64_t t = tickets_take(a.ws, next, &sh.tile, a.io.ntiles);
+        if (t >= a.io.ntiles) break;
+        root_tile<I>(a.g, a.io, a.nchk, a.chk_rel, t, t, 0, a.ws, ep, sh.sm);
     }
     finish_launch(a.ws);
 }
@@ -367,169 +466,525 @@ DEV uint64_t frontier_search(const uint64_t *__restrict__ off, uint64_t S, uint6
     return lo;
 }
 
+// Expand tile `lt` of a step (global tile gt; the step's scan starts at gt0).
+// `dyn` is the dynamic LDS: the node-start bitmap, the per-position generator
+// base and a pool of pool_words words that stages, per frontier node of the
+// tile, the checked ancestor segments, the next generator segment and the
+// placed vertices, as far as they fit.  LAST_COUNT adds its embeddings and
+// digest to the thread's accumulators; the other kinds write their output.
+template <uint32_t KIND, uint32_t I>
+DEV void expand_tile(const DevGraphView &g, const DepthPlanDev &dp, const uint32_t *col, const uint32_t *dcol,
+                     const StepIO &io, uint64_t lt, uint64_t gt, uint64_t gt0, const ScanWS &ws, uint32_t ep,
+                     uint32_t pool_words, TileShared<I> &sh, unsigned char *dyn, uint64_t &acc_cnt,
+                     uint64_t &acc_h0, uint64_t &acc_h1)
+{
+    constexpr uint32_t T = kTile, TC = kTile * I, NW = TC / 64;
+    const uint32_t d = io.d;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = lane_id(), wave = tid >> 6;
+    uint64_t *s_bits = reinterpret_cast<uint64_t *>(dyn);            // [NW] node start positions
+    uint32_t *s_seg = reinterpret_cast<uint32_t *>(s_bits + NW);     // [TC] nbr index of tile position 0
+    uint32_t *s_pool = s_seg + TC;                                   // [pool_words]
+
+    const uint64_t c0 = io.g_begin + lt * TC;
+    const uint64_t c1 = (c0 + TC < io.g_end) ? c0 + TC : io.g_end;
+    const uint32_t n = (uint32_t)(c1 - c0);
+
+    // Frontier nodes covering [c0, c1): offsets are strictly increasing.
+    if (tid < NW) s_bits[tid] = 0;
+    if (tid < 64) {
+        const uint32_t sub = tid & 31;
+        const uint64_t key = (tid < 32) ? c0 : c1 - 1;
+        const uint64_t i = frontier_search(io.in_off, io.S_prev, key, sub);
+        if (sub == 0) sh.node[tid >> 5] = i;
+    }
+    __syncthreads();
+    const uint64_t ilo = sh.node[0];
+    const uint32_t nf = (uint32_t)(sh.node[1] - ilo + 1);
+    const uint32_t nchk = dp.nchk;
+    const uint32_t need_nxt = (KIND == EXPAND_MID) ? io.need_nxt : 0u;
+    const uint32_t meta_words = 2 * nf * nchk + (need_nxt ? 2 * nf : 0);
+    const bool stage_meta = meta_words <= pool_words;
+    const bool staged = stage_meta && meta_words + nf * d <= pool_words;
+    uint2 *s_chk = reinterpret_cast<uint2 *>(s_pool);  // [nchk][nf]
+    uint2 *s_nxt = s_chk + nchk * nf;                   // [nf]
+    uint32_t *s_rows = s_pool + meta_words;             // [nf][d]
+    const uint32_t *in_rows = io.in_vec + ilo * d;      // the tile's node rows, contiguous
+
+    if (staged)
+        for (uint32_t u = tid; u < nf * d; u += T) s_rows[u] = in_rows[u];
+    // Per frontier node: start bit, generator base, staged segments.
+    for (uint32_t j = tid; j < nf; j += T) {
+        const uint64_t i = ilo + j;
+        const uint64_t o = io.in_off[i];
+        const uint32_t first = (o > c0) ? (uint32_t)(o - c0) : 0u;
+        atomicOr((unsigned long long *)&s_bits[first >> 6], 1ull << (first & 63));
+        s_seg[j] = io.in_gbeg[i] + (uint32_t)(c0 - o);
+        if (stage_meta) {
+            const uint32_t *av = in_rows + (uint64_t)j * d;
+            for (uint32_t r = 0; r < nchk; r++) {
+                const uint32_t dist = dp.chk_dist[r];
+                if (dist) {
+                    uint32_t x[1] = {av[d - dist]};
+                    bool act[1] = {true};
+                    uint2 sg[1];
+                    seg_lookup_n<1>(g, x, dp.chk_rel[r], act, sg);
+                    s_chk[r * nf + j] = sg[0];
+                }
+            }
+            if (need_nxt) {
+                uint32_t x[1] = {av[d + 1 - dp.nxt_dist]};
+                bool act[1] = {true};
+                uint2 sg[1];
+                seg_lookup_n<1>(g, x, dp.nxt_rel, act, sg);
+                s_nxt[j] = sg[0];
+            }
+        }
+    }
+    __syncthreads();
+
+    // Bitmap words of this wave (word k * 4 + wave covers its 64 positions
+    // of item k) and the set bits before each of them.
+    uint32_t wpre[I];
+    uint64_t wbits[I];
+    {
+        uint32_t acc = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < I; k++)
+#pragma unroll
+            for (uint32_t w = 0; w < kWaves; w++) {
+                const uint64_t b = s_bits[k * kWaves + w];
+                if (w == wave) {
+                    wpre[k] = acc;
+                    wbits[k] = b;
+                }
+                acc += (uint32_t)__popcll(b);
+            }
+    }
+
+    // Phase 1: node, generator position, candidate read (all items).
+    uint32_t jj[I], w[I], cnt[I], gb[I];
+    bool keep[I];
+#pragma unroll
+    for (uint32_t k = 0; k < I; k++) {
+        const uint32_t x = k * T + tid;
+        const bool valid = x < n;
+        const uint32_t j = wpre[k] + mask_prefix(wbits[k]) + (uint32_t)((wbits[k] >> lane) & 1ull) - 1u;
+        jj[k] = valid ? j : 0u;
+        const uint32_t pos = s_seg[jj[k]] + x;
+        keep[k] = valid && pos < g.nent;  // offsets and segments always agree; never read past the array
+        w[k] = keep[k] ? g.nbr[pos] : 0u;
+        cnt[k] = 0;
+        gb[k] = 0;
+    }
+    // Phase 2: injectivity against the node's placed vertices.
+    {
+        const uint32_t *rows = staged ? s_rows : in_rows;
+#pragma unroll
+        for (uint32_t k = 0; k < I; k++) {
+            const uint32_t *av = rows + jj[k] * d;
+            for (uint32_t kk = 0; kk < d; kk++) keep[k] &= (av[kk] != w[k]);
+        }
+    }
+    // Phase 3: membership in every checked segment, I searches in lockstep.
+    for (uint32_t r = 0; r < nchk; r++) {
+        const uint32_t dist = dp.chk_dist[r];
+        uint2 sg[I];
+        if (dist && stage_meta) {
+#pragma unroll
+            for (uint32_t k = 0; k < I; k++) sg[k] = s_chk[r * nf + jj[k]];
+        } else {
+            uint32_t x[I];
+            const uint32_t *rows = staged ? s_rows : in_rows;
+#pragma unroll
+            for (uint32_t k = 0; k < I; k++) x[k] = dist ? rows[jj[k] * d + (d - dist)] : w[k];
+            seg_lookup_n<I>(g, x, dp.chk_rel[r], keep, sg);
+        }
+        seg_contains_n<I>(g.nbr, sg, w, keep);
+    }
+
+    if (KIND == EXPAND_MID) {
+        // Phase 4: the survivor's next generator segment; no children, no node.
+        uint2 ns[I];
+        if (need_nxt && stage_meta) {
+#pragma unroll
+            for (uint32_t k = 0; k < I; k++) ns[k] = s_nxt[jj[k]];
+        } else {
+            uint32_t x[I];
+            const uint32_t *rows = staged ? s_rows : in_rows;
+#pragma unroll
+            for (uint32_t k = 0; k < I; k++) x[k] = need_nxt ? rows[jj[k] * d + (d + 1 - dp.nxt_dist)] : w[k];
+            seg_lookup_n<I>(g, x, dp.nxt_rel, keep, ns);
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < I; k++) {
+            if (!keep[k]) continue;
+            cnt[k] = ns[k].y - ns[k].x;
+            gb[k] = ns[k].x;
+            keep[k] = cnt[k] > 0;
+        }
+    }
+
+    if (KIND != EXPAND_LAST_COUNT) {
+        // Compaction: wave-level ballot/mbcnt and DPP scans, then the
+        // tile's prefix from the look-back, then each item's slot.
+        ScanSmem<I> &sm = sh.sm;
+        uint32_t kpre[I], cpre[I];
+#pragma unroll
+        for (uint32_t k = 0; k < I; k++) {
+            const uint64_t m = __ballot(keep[k]);
+            kpre[k] = mask_prefix(m);
+            const uint32_t cinc = wave_incl_scan(cnt[k]);
+            cpre[k] = cinc - cnt[k];
+            if (lane == 63) {
+                sm.wk[k][wave] = (uint32_t)__popcll(m);
+                sm.wc[k][wave] = cinc;
+            }
+        }
+        __syncthreads();
+        tile_prefix<I>(gt, gt0, ws, ep, sm);
+        uint64_t bk = sm.ex[0], bc = sm.ex[1];
+        const uint32_t *rows = staged ? s_rows : in_rows;
+#pragma unroll
+        for (uint32_t k = 0; k < I; k++) {
+            uint64_t ok_ = bk, oc_ = bc;
+#pragma unroll
+            for (uint32_t ww = 0; ww < kWaves; ww++) {
+                if (ww < wave) {
+                    ok_ += sm.wk[k][ww];
+                    oc_ += sm.wc[k][ww];
+                }
+                bk += sm.wk[k][ww];
+                bc += sm.wc[k][ww];
+            }
+            if (!keep[k]) continue;
+            const uint64_t idx = ok_ + kpre[k];
+            if (idx >= io.cap) continue;
+            const uint32_t *av = rows + jj[k] * d;
+            if (KIND == EXPAND_MID) {
+                uint32_t *ov = io.out_vec + idx * (d + 1);
+                for (uint32_t kk = 0; kk < d; kk++) ov[kk] = av[kk];
+                ov[d] = w[k];
+                io.out_off[idx] = oc_ + cpre[k];
+                io.out_gbeg[idx] = gb[k];
+            } else {
+                uint32_t *row = io.rows + idx * io.q;
+                for (uint32_t kk = 0; kk < d; kk++) row[col[kk]] = av[kk];
+                row[col[d]] = w[k];
+            }
+        }
+        if (lt == io.ntiles - 1 && tid == 0) {
+            io.tot->keep = sm.tot[0];
+            io.tot->cnt = sm.tot[1];
+            if (KIND == EXPAND_MID && sm.tot[0] <= io.cap) io.out_off[sm.tot[0]] = sm.tot[1];
+        }
+    } else {
+#pragma unroll
+        for (uint32_t k = 0; k < I; k++) acc_cnt += keep[k] ? 1u : 0u;
+    }
+    if (KIND != EXPAND_MID && io.digest) {
+        const uint32_t *rows = staged ? s_rows : in_rows;
+#pragma unroll
+        for (uint32_t k = 0; k < I; k++) {
+            if (!keep[k]) continue;
+            uint64_t x0 = 0x9E3779B97F4A7C15ull, x1 = 0xD1B54A32D192ED03ull;
+            for (uint32_t c = 0; c < io.q; c++) {
+                const uint32_t dc = dcol[c];
+                const uint32_t val = (dc == d) ? w[k] : rows[jj[k] * d + dc];
+                const uint64_t xv = (uint64_t)val | ((uint64_t)c << 32);
+                x0 = mix64(x0 ^ xv);
+                x1 = mix64(x1 ^ xv);
+            }
+            acc_h0 += x0;
+            acc_h1 += x1;
+        }
+    }
+}
+
 template <uint32_t KIND, uint32_t I>
 __global__ __launch_bounds__(kTile) void k_expand(ExpandArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ ScanSmem<I> sm;
-    __shared__ uint64_t s_tile;
-    __shared__ uint64_t s_node[2];
-    __shared__ uint64_t s_red[3][kWaves];
-    constexpr uint32_t T = kTile, TC = kTile * I, NW = TC / 64;
-    const uint32_t d = a.d;
-    const uint32_t tid = threadIdx.x;
-    const uint32_t lane = lane_id(), wave = tid >> 6;
-
-    // LDS staging, sized for TC frontier nodes (a tile never spans more).
-    // s_bits marks the tile positions where a node's candidates start: the
-    // node of position x is (set bits at positions <= x) - 1.
-    uint64_t *s_bits = reinterpret_cast<uint64_t *>(smem);                // [NW]
-    uint32_t *s_seg = reinterpret_cast<uint32_t *>(s_bits + NW);          // [TC] nbr index of tile position 0
-    uint2 *s_chk = reinterpret_cast<uint2 *>(s_seg + TC);                 // [nchk][TC] checked segments
-    uint2 *s_nxt = s_chk + a.dp.nchk * TC;                                // [TC] next generator (need_nxt)
-
+    __shared__ TileShared<I> sh;
     const uint32_t ep = read_epoch(a.ws);
     uint64_t acc_cnt = 0, acc_h0 = 0, acc_h1 = 0;
     uint32_t next = 0;
     tickets_init(a.ws, next);
     for (;;) {
-        const uint64_t t = tickets_take(a.ws, next, &s_tile, a.ntiles);
-        if (t >= a.ntiles) break;
-        const uint64_t c0 = a.g_begin + t * TC;
-        const uint64_t c1 = (c0 + TC < a.g_end) ? c0 + TC : a.g_end;
-        const uint32_t n = (uint32_t)(c1 - c0);
+        const uint64_t t = tickets_take(a.ws, next, &sh.tile, a.io.ntiles);
+        if (t >= a.io.ntiles) break;
+        expand_tile<KIND, I>(a.g, a.dp, a.col, a.dcol, a.io, t, t, 0, a.ws, ep, a.pool_words, sh, smem, acc_cnt,
+                             acc_h0, acc_h1);
+    }
+    if (KIND != EXPAND_MID) flush_sums(acc_cnt, acc_h0, acc_h1, a.io.tot, sh.red);
+    finish_launch(a.ws);
+}
 
-        // Frontier nodes covering [c0, c1): offsets are strictly increasing.
-        if (tid < NW) s_bits[tid] = 0;
-        if (tid < 64) {
-            const uint32_t sub = tid & 31;
-            const uint64_t key = (tid < 32) ? c0 : c1 - 1;
-            const uint64_t i = frontier_search(a.in_off, a.S_prev, key, sub);
-            if (sub == 0) s_node[tid >> 5] = i;
+// ---------------------------------------------------------------------------
+// batched matching: device scheduler + persistent step kernel
+// ---------------------------------------------------------------------------
+
+DEV uint64_t umin64(uint64_t a, uint64_t b) { return a < b ? a : b; }
+DEV uint64_t umax64(uint64_t a, uint64_t b) { return a > b ? a : b; }
+
+// Consume the finished step of slot B (depth-first chunk walk), then
+// schedule its next step.  Returns the new step's tile count (0 when the slot
+// has no step).  Mirrors the single-query driver's recursion: a level that
+// overflows its buffer is retried on half the range; otherwise the chunk is
+// re-estimated from the observed selectivity.
+DEV uint32_t schedule_slot(const BatchCfg &cfg, BatchSlot &B)
+{
+    const uint64_t TC = (uint64_t)kTile * cfg.items;
+    const uint64_t by_val = kValueMask / umax64(cfg.max_degree, 1);
+    const uint64_t step = cfg.ctl->step_cand ? umin64(cfg.ctl->step_cand, cfg.max_step_cand) : cfg.max_step_cand;
+    const uint64_t limit = umax64(TC, umin64(step, by_val));
+    const uint32_t q = B.q;
+    uint32_t state = B.state;
+    if (state == SLOT_NEW) {
+        B.root_next = 0;
+        for (uint32_t k = 0; k < GFSM_MAXQ; k++) {
+            B.lo[k] = B.hi[k] = B.S[k] = 0;
+            B.level[k] = B.cand[k] = 0;
+            B.chunk[k] = (k + 1 < q) ? umin64(B.cap[k], limit) : limit;
         }
-        __syncthreads();
-        const uint64_t ilo = s_node[0];
-        const uint32_t nf = (uint32_t)(s_node[1] - ilo + 1);
-
-        // Per frontier node: start bit, generator base, checked segments.
-        for (uint32_t j = tid; j < nf; j += T) {
-            const uint64_t i = ilo + j;
-            const uint64_t o = a.in_off[i];
-            const uint32_t first = (o > c0) ? (uint32_t)(o - c0) : 0u;
-            atomicOr((unsigned long long *)&s_bits[first >> 6], 1ull << (first & 63));
-            s_seg[j] = a.in_gbeg[i] + (uint32_t)(c0 - o);
-            const uint32_t *av = a.in_vec + i * d;
-            for (uint32_t r = 0; r < a.dp.nchk; r++) {
-                const uint32_t dist = a.dp.chk_dist[r];
-                if (dist) s_chk[r * TC + j] = seg_lookup(a.g, av[d - dist], a.dp.chk_rel[r]);
+        B.chunk[0] = umax64(TC, umin64(B.cap[0], limit));
+        B.cand_done = 0;
+        B.count = B.dg0 = B.dg1 = 0;
+        B.steps = B.overflows = 0;
+        B.error = 0;
+        B.step = STEP_NONE;
+        state = SLOT_RUN;
+    } else if (state == SLOT_RUN) {
+        const Totals t = B.tot;
+        const uint64_t b = B.cur_lo, e = B.cur_hi;
+        const uint32_t d = B.d;
+        if (B.step == STEP_ROOT) {
+            B.root_next = e;
+            B.level[0] += t.keep;
+            if (t.keep) {
+                B.S[0] = t.keep;
+                B.lo[1] = 0;
+                B.hi[1] = t.cnt;
             }
-            if (KIND == EXPAND_MID && a.need_nxt) s_nxt[j] = seg_lookup(a.g, av[d + 1 - a.dp.nxt_dist], a.dp.nxt_rel);
-        }
-        __syncthreads();
-
-        bool keep[I];
-        uint32_t cnt[I], w[I], gb[I];
-        uint64_t node[I];
-#pragma unroll
-        for (uint32_t k = 0; k < I; k++) {
-            const uint32_t x = k * T + tid;
-            keep[k] = false;
-            cnt[k] = 0;
-            w[k] = 0;
-            gb[k] = 0;
-            node[k] = ilo;
-            if (x < n) {
-                // word k * 4 + wave covers this wave's 64 positions of item k
-                const uint32_t wi = k * kWaves + wave;
-                const uint64_t bits = s_bits[wi];
-                uint32_t pre = 0;
-                for (uint32_t u = 0; u < wi; u++) pre += (uint32_t)__popcll(s_bits[u]);
-                const uint32_t j = pre + mask_prefix(bits) + (uint32_t)((bits >> lane) & 1ull) - 1u;
-                node[k] = ilo + j;
-                const uint32_t pos = s_seg[j] + x;
-                bool ok = pos < a.g.nent;  // offsets and segments always agree; never read past the array
-                const uint32_t wv = ok ? a.g.nbr[pos] : 0u;
-                w[k] = wv;
-                const uint32_t *av = a.in_vec + node[k] * d;
-                for (uint32_t kk = 0; kk < d; kk++) ok &= (av[kk] != wv);
-                for (uint32_t r = 0; r < a.dp.nchk && ok; r++) {
-                    const uint2 sg = a.dp.chk_dist[r] ? s_chk[r * TC + j] : seg_lookup(a.g, wv, a.dp.chk_rel[r]);
-                    ok = seg_contains(a.g.nbr, sg.x, sg.y, wv);
-                }
-                if (KIND == EXPAND_MID) {
-                    if (ok) {
-                        const uint2 ns = a.need_nxt ? s_nxt[j] : seg_lookup(a.g, wv, a.dp.nxt_rel);
-                        cnt[k] = ns.y - ns.x;
-                        gb[k] = ns.x;
-                        keep[k] = cnt[k] > 0;
-                    }
-                } else {
-                    keep[k] = ok;
-                }
-            }
-        }
-
-        if (KIND == EXPAND_MID) {
-            uint64_t kx[I], off[I], rk, rc;
-            tile_scan<I>(keep, cnt, t, a.ws, ep, sm, kx, off, rk, rc);
-            const uint32_t wd = d + 1;
-#pragma unroll
-            for (uint32_t k = 0; k < I; k++) {
-                if (keep[k] && kx[k] < a.cap) {
-                    const uint32_t *av = a.in_vec + node[k] * d;
-                    uint32_t *ov = a.out_vec + kx[k] * wd;
-                    for (uint32_t kk = 0; kk < d; kk++) ov[kk] = av[kk];
-                    ov[d] = w[k];
-                    a.out_off[kx[k]] = off[k];
-                    a.out_gbeg[kx[k]] = gb[k];
-                }
-            }
-            if (t == a.ntiles - 1 && tid == 0) {
-                a.tot->keep = rk;
-                a.tot->cnt = rc;
-                if (rk <= a.cap) a.out_off[rk] = rc;
-            }
-        } else {
-            if (KIND == EXPAND_LAST_ROWS) {
-                uint64_t kx[I], off[I], rk, rc;
-                const uint32_t zero[I] = {};
-                tile_scan<I>(keep, zero, t, a.ws, ep, sm, kx, off, rk, rc);
-#pragma unroll
-                for (uint32_t k = 0; k < I; k++) {
-                    if (keep[k] && kx[k] < a.cap) {
-                        const uint32_t *av = a.in_vec + node[k] * d;
-                        uint32_t *row = a.rows + kx[k] * a.q;
-                        for (uint32_t kk = 0; kk < d; kk++) row[a.col[kk]] = av[kk];
-                        row[a.col[d]] = w[k];
-                    }
-                }
-                if (t == a.ntiles - 1 && tid == 0) a.tot->keep = rk;
+            B.steps++;
+        } else if (B.step == EXPAND_MID) {
+            B.steps++;
+            if (t.keep > B.cap[d]) {
+                B.overflows++;
+                B.chunk[d] = umax64(TC, (e - b) / 2);
             } else {
-#pragma unroll
-                for (uint32_t k = 0; k < I; k++) acc_cnt += keep[k] ? 1u : 0u;
-            }
-            if (a.digest) {
-#pragma unroll
-                for (uint32_t k = 0; k < I; k++) {
-                    if (!keep[k]) continue;
-                    const uint32_t *av = a.in_vec + node[k] * d;
-                    uint64_t x0 = 0x9E3779B97F4A7C15ull, x1 = 0xD1B54A32D192ED03ull;
-                    for (uint32_t c = 0; c < a.q; c++) {
-                        const uint32_t dc = a.dcol[c];
-                        const uint32_t val = (dc == d) ? w[k] : av[dc];
-                        const uint64_t xv = (uint64_t)val | ((uint64_t)c << 32);
-                        x0 = mix64(x0 ^ xv);
-                        x1 = mix64(x1 ^ xv);
-                    }
-                    acc_h0 += x0;
-                    acc_h1 += x1;
+                B.lo[d] = e;
+                B.cand[d] += e - b;
+                B.level[d] += t.keep;
+                if (t.keep) {
+                    const uint64_t est = B.cap[d] / 10 * 7 * (e - b) / t.keep;
+                    B.chunk[d] = umax64(TC, umin64(est, limit));
+                    B.S[d] = t.keep;
+                    B.lo[d + 1] = 0;
+                    B.hi[d + 1] = t.cnt;
+                } else {
+                    B.chunk[d] = umin64(B.chunk[d] * 4, limit);
                 }
             }
-            __syncthreads();  // LDS staging is rewritten by the next tile
+        } else if (B.step == EXPAND_LAST_COUNT) {
+            B.steps++;
+            B.lo[d] = e;
+            B.cand[d] += e - b;
+            B.level[d] += t.count;
+            B.count += t.count;
+            B.dg0 += t.dg0;
+            B.dg1 += t.dg1;
+        }
+    } else {
+        B.step = STEP_NONE;
+        return 0;
+    }
+
+    // Next step: the deepest depth with candidates left, else more roots.
+    int32_t d = (int32_t)q - 1;
+    while (d >= 1 && B.lo[d] >= B.hi[d]) d--;
+    uint32_t ntiles = 0;
+    B.tot = Totals{};
+    if (d >= 1) {
+        const uint32_t kind = (d == (int32_t)q - 1) ? (uint32_t)EXPAND_LAST_COUNT : (uint32_t)EXPAND_MID;
+        const uint64_t b = B.lo[d];
+        const uint64_t e = umin64(B.hi[d], b + umax64(TC, umin64(B.chunk[d], limit)));
+        const DepthPlanDev &dp = B.plan->depth[d];
+        if (B.S[d - 1] == 0 || B.S[d - 1] > B.cap[d - 1] || dp.gen_dist == 0 || dp.gen_dist > (uint32_t)d) {
+            B.state = SLOT_ERROR;
+            B.error = 1;
+            B.step = STEP_NONE;
+            return 0;
+        }
+        if (B.max_cand && B.cand_done + (e - b) > B.max_cand) {
+            B.state = SLOT_CAP;
+            B.step = STEP_NONE;
+            return 0;
+        }
+        B.cand_done += e - b;
+        StepIO io{};
+        io.in_vec = B.vec[d - 1];
+        io.in_off = B.off[d - 1];
+        io.in_gbeg = B.gbeg[d - 1];
+        io.S_prev = B.S[d - 1];
+        io.g_begin = b;
+        io.g_end = e;
+        if (kind == EXPAND_MID) {
+            io.out_vec = B.vec[d];
+            io.out_off = B.off[d];
+            io.out_gbeg = B.gbeg[d];
+            io.cap = B.cap[d];
+            io.need_nxt = dp.nxt_dist > 1 ? 1u : 0u;
+        }
+        io.tot = &B.tot;
+        io.ntiles = (e - b + TC - 1) / TC;
+        io.d = (uint32_t)d;
+        io.q = q;
+        io.kind = kind;
+        io.digest = (kind == EXPAND_LAST_COUNT) ? B.digest : 0u;
+        B.io = io;
+        B.step = kind;
+        B.d = (uint32_t)d;
+        B.cur_lo = b;
+        B.cur_hi = e;
+        ntiles = (uint32_t)io.ntiles;
+    } else if (B.root_next < B.nroots) {
+        const uint64_t b = B.root_next;
+        const uint64_t e = umin64(B.nroots, b + B.chunk[0]);
+        RootIO r{};
+        r.cand_vtx = B.root_vtx + b;
+        r.cand_beg = B.root_beg + b;
+        r.cand_cnt = B.root_cnt + b;
+        r.n = e - b;
+        r.out_vec = B.vec[0];
+        r.out_off = B.off[0];
+        r.out_gbeg = B.gbeg[0];
+        r.cap = B.cap[0];
+        r.tot = &B.tot;
+        r.ntiles = (e - b + TC - 1) / TC;
+        B.rio = r;
+        B.step = STEP_ROOT;
+        B.d = 0;
+        B.cur_lo = b;
+        B.cur_hi = e;
+        ntiles = (uint32_t)r.ntiles;
+    } else {
+        state = SLOT_DONE;
+        B.step = STEP_NONE;
+    }
+    B.state = state;
+    return ntiles;
+}
+
+// One workgroup: schedule every slot, then lay the slots' tiles end to end.
+__global__ __launch_bounds__(kTile) void k_batch_schedule(BatchCfg cfg)
+{
+    __shared__ uint32_t s_wt[kWaves], s_wr[kWaves];
+    const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+    const uint32_t per = (cfg.nslots + kTile - 1) / kTile;  // <= kMaxSlots / kTile
+    const uint32_t s0 = tid * per;
+    uint32_t nt[kMaxSlots / kTile];
+    uint32_t my_tiles = 0, my_run = 0;
+    for (uint32_t u = 0; u < per; u++) {
+        nt[u] = 0;
+        const uint32_t s = s0 + u;
+        if (s < cfg.nslots) {
+            BatchSlot &B = cfg.slots[s];
+            nt[u] = schedule_slot(cfg, B);
+            my_tiles += nt[u];
+            my_run += (B.state == SLOT_RUN) ? 1u : 0u;
+            SlotReport rp;
+            rp.state = B.state;
+            rp.steps = B.steps;
+            rp.count = B.count;
+            rp.dg0 = B.dg0;
+            rp.dg1 = B.dg1;
+            cfg.report[s] = rp;
         }
     }
-    if (KIND != EXPAND_MID) flush_sums(acc_cnt, acc_h0, acc_h1, a.tot, s_red);
-    finish_launch(a.ws);
+    const uint32_t inc = wave_incl_scan(my_tiles);
+    const uint32_t rinc = wave_incl_scan(my_run);
+    if (lane == 63) {
+        s_wt[wave] = inc;
+        s_wr[wave] = rinc;
+    }
+    __syncthreads();
+    uint32_t base = inc - my_tiles, total = 0, running = 0;
+    for (uint32_t w = 0; w < kWaves; w++) {
+        if (w < wave) base += s_wt[w];
+        total += s_wt[w];
+        running += s_wr[w];
+    }
+    for (uint32_t u = 0; u < per; u++) {
+        const uint32_t s = s0 + u;
+        if (s < cfg.nslots) {
+            cfg.tile_start[s] = base;
+            cfg.slots[s].tile0 = base;
+            base += nt[u];
+        }
+    }
+    if (tid == 0) {
+        cfg.tile_start[cfg.nslots] = total;
+        cfg.ctl->total_tiles = total;
+        cfg.ctl->running = running;
+    }
+}
+
+// Slot owning global tile t: the last s in [0, nslots) with start[s] <= t
+// (empty slots share their start with the next slot), by one wave in 64-ary
+// steps.
+DEV uint32_t slot_search(const uint64_t *__restrict__ start, uint32_t nslots, uint64_t t)
+{
+    uint32_t lo = 0, hi = nslots;
+    while (hi - lo > 1) {
+        const uint32_t n = hi - lo;
+        const uint32_t stride = (n + 63) / 64;
+        const uint32_t pos = lo + lane_id() * stride;
+        const bool ok = pos < hi && start[pos] <= t;
+        const uint64_t m = __ballot(ok);
+        const uint32_t top = 63u - (uint32_t)__builtin_clzll(m | 1ull);
+        lo = lo + top * stride;
+        hi = (lo + stride < hi) ? lo + stride : hi;
+    }
+    return lo;
+}
+
+template <uint32_t I>
+__global__ __launch_bounds__(kTile) void k_batch_step(BatchCfg cfg)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ TileShared<I> sh;
+    const uint64_t total = cfg.ctl->total_tiles;
+    const uint32_t ep = read_epoch(cfg.ws);
+    uint32_t next = 0;
+    tickets_init(cfg.ws, next);
+    for (;;) {
+        const uint64_t t = tickets_take(cfg.ws, next, &sh.tile, total);
+        if (t >= total) break;
+        if (threadIdx.x < 64) {
+            const uint32_t s = slot_search(cfg.tile_start, cfg.nslots, t);
+            if (threadIdx.x == 0) sh.slot = s;
+        }
+        __syncthreads();
+        const uint32_t s = __builtin_amdgcn_readfirstlane(sh.slot);
+        const BatchSlot &B = cfg.slots[s];
+        const uint32_t step = B.step;
+        const uint64_t t0 = B.tile0;
+        const QueryPlanDev *plan = B.plan;
+        if (step == STEP_ROOT) {
+            root_tile<I>(cfg.g, B.rio, plan->nroot_chk, plan->root_chk_rel, t - t0, t, t0, cfg.ws, ep, sh.sm);
+        } else if (step == EXPAND_MID) {
+            uint64_t c = 0, h0 = 0, h1 = 0;
+            expand_tile<EXPAND_MID, I>(cfg.g, plan->depth[B.d], plan->col, plan->dcol, B.io, t - t0, t, t0, cfg.ws,
+                                       ep, cfg.pool_words, sh, smem, c, h0, h1);
+        } else {
+            uint64_t c = 0, h0 = 0, h1 = 0;
+            expand_tile<EXPAND_LAST_COUNT, I>(cfg.g, plan->depth[B.d], plan->col, plan->dcol, B.io, t - t0, t, t0,
+                                              cfg.ws, ep, cfg.pool_words, sh, smem, c, h0, h1);
+            flush_sums(c, h0, h1, B.io.tot, sh.red);
+        }
+    }
+    finish_launch(cfg.ws);
 }
 
 // ---------------------------------------------------------------------------
@@ -540,13 +995,12 @@ __global__ __launch_bounds__(kTile) void k_scan_test(const uint32_t *vals, uint6
                                                     uint32_t *idx, ScanWS ws, Totals *tot, uint64_t ntiles)
 {
     constexpr uint32_t I = kScanTestItems;
-    __shared__ ScanSmem<I> sm;
-    __shared__ uint64_t s_tile;
+    __shared__ TileShared<I> sh;
     const uint32_t ep = read_epoch(ws);
     uint32_t next = 0;
     tickets_init(ws, next);
     for (;;) {
-        const uint64_t t = tickets_take(ws, next, &s_tile, ntiles);
+        const uint64_t t = tickets_take(ws, next, &sh.tile, ntiles);
         if (t >= ntiles) break;
         bool keep[I];
         uint32_t v[I];
@@ -557,7 +1011,7 @@ __global__ __launch_bounds__(kTile) void k_scan_test(const uint32_t *vals, uint6
             keep[k] = v[k] != 0;
         }
         uint64_t kx[I], off[I], rk, rc;
-        tile_scan<I>(keep, v, t, ws, ep, sm, kx, off, rk, rc);
+        tile_scan<I>(keep, v, t, 0, ws, ep, sh.sm, kx, off, rk, rc);
 #pragma unroll
         for (uint32_t k = 0; k < I; k++) {
             const uint64_t i = t * (kTile * I) + k * kTile + threadIdx.x;
@@ -591,27 +1045,31 @@ __global__ void k_ws_clear(ScanWS ws)
 // launch wrappers
 // ---------------------------------------------------------------------------
 
-size_t expand_lds_bytes(uint32_t nchk, uint32_t need_nxt, uint32_t items)
+size_t expand_lds_bytes(uint32_t items, uint32_t pool_words)
 {
     const size_t TC = (size_t)kTile * items;
-    size_t b = TC / 64 * 8;                 // s_bits
-    b += TC * 4;                            // s_seg
-    b += (size_t)nchk * TC * 8;             // s_chk
-    if (need_nxt) b += TC * 8;              // s_nxt
-    return b;
+    return TC / 64 * 8 + TC * 4 + (size_t)pool_words * 4;  // bitmap, generator bases, pool
+}
+
+// Raise a kernel's dynamic LDS limit once (before any launch or capture).
+template <class K>
+static hipError_t allow_lds(K kernel, bool &done)
+{
+    if (done) return hipSuccess;
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxDynLds);
+    if (e == hipSuccess) done = true;
+    return e;
 }
 
 template <uint32_t KIND, uint32_t I>
 static hipError_t launch_expand_t(const ExpandArgs &a, uint32_t grid, hipStream_t s)
 {
-    const size_t lds = expand_lds_bytes(a.dp.nchk, a.need_nxt, I);
-    static bool attr_set = false;
-    if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_expand<KIND, I>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024);
-        if (e != hipSuccess) return e;
-        attr_set = true;
-    }
+    static bool attr = false;
+    const size_t lds = expand_lds_bytes(I, a.pool_words);
+    if (lds > kMaxDynLds) return hipErrorInvalidValue;
+    hipError_t e = allow_lds(&k_expand<KIND, I>, attr);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_expand<KIND, I>), dim3(grid), dim3(kTile), lds, s, a);
     return hipGetLastError();
 }
@@ -630,7 +1088,7 @@ static hipError_t launch_expand_k(const ExpandArgs &a, uint32_t grid, hipStream_
 
 hipError_t launch_expand(const ExpandArgs &a, uint32_t grid, hipStream_t s)
 {
-    switch (a.kind) {
+    switch (a.io.kind) {
     case EXPAND_MID: return launch_expand_k<EXPAND_MID>(a, grid, s);
     case EXPAND_LAST_COUNT: return launch_expand_k<EXPAND_LAST_COUNT>(a, grid, s);
     default: return launch_expand_k<EXPAND_LAST_ROWS>(a, grid, s);
@@ -641,6 +1099,37 @@ hipError_t launch_root(const RootArgs &a, uint32_t grid, hipStream_t s)
 {
     hipLaunchKernelGGL((k_root<kRootItems>), dim3(grid), dim3(kTile), 0, s, a);
     return hipGetLastError();
+}
+
+hipError_t launch_batch_schedule(const BatchCfg &cfg, hipStream_t s)
+{
+    if (cfg.nslots == 0 || cfg.nslots > kMaxSlots) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_batch_schedule, dim3(1), dim3(kTile), 0, s, cfg);
+    return hipGetLastError();
+}
+
+template <uint32_t I>
+static hipError_t launch_batch_step_t(const BatchCfg &cfg, uint32_t grid, hipStream_t s)
+{
+    static bool attr = false;
+    const size_t lds = expand_lds_bytes(I, cfg.pool_words);
+    if (lds > kMaxDynLds) return hipErrorInvalidValue;
+    hipError_t e = allow_lds(&k_batch_step<I>, attr);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_batch_step<I>), dim3(grid), dim3(kTile), lds, s, cfg);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_step(const BatchCfg &cfg, uint32_t grid, hipStream_t s)
+{
+    if (cfg.nslots == 0 || cfg.nslots > kMaxSlots) return hipErrorInvalidValue;
+    switch (cfg.items) {
+    case 1: return launch_batch_step_t<1>(cfg, grid, s);
+    case 2: return launch_batch_step_t<2>(cfg, grid, s);
+    case 4: return launch_batch_step_t<4>(cfg, grid, s);
+    case 8: return launch_batch_step_t<8>(cfg, grid, s);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_scan_test(const uint32_t *vals, uint64_t n, uint64_t *out, uint32_t *idx, ScanWS ws,

@@ -1,4 +1,15 @@
-// serve.cpp -- placeholder serving loop: one worker, FIFO over the plain path.
+// serve.cpp -- the serving loop: a resident data graph, queries submitted at
+// any time, continuous batching (no reference counterpart; the reference is
+// one process per query, main.cu:56-63).
+//
+// One worker thread owns a BatchEngine.  Between two replays of the captured
+// steps it harvests the slots that finished and admits waiting queries in
+// arrival order while a slot is free and the arena has room for the query's
+// projected intermediate levels; a query that does not fit waits at the head
+// of the queue (no overtaking, so no starvation).  Admitted queries join the
+// steps already in flight, so their work overlaps on the device with every
+// other query's.  Results do not depend on what runs beside them: each slot
+// has its own levels, its own scan range and its own totals.
 #include "serve.h"
 
 #include <chrono>
@@ -9,78 +20,162 @@ struct Server {
     Ctx *ctx = nullptr;
     const HostGraph *data = nullptr;
     gfsm_serve_opts opts{};
+    std::unique_ptr<BatchEngine> eng;
     std::mutex mu;
-    std::condition_variable cv;
+    std::condition_variable cv_done, cv_work;
     struct Item {
-        const HostGraph *q;
-        double t[3];
+        double t[3] = {0, 0, 0};  // submit, admit, done (seconds on the server clock)
         bool done = false;
         Result res;
     };
     std::map<uint64_t, Item> items;
-    std::deque<uint64_t> queue;
+    std::deque<std::unique_ptr<BatchJob>> incoming;  // submitted, not yet seen by the worker
     uint64_t next = 1;
     bool stop = false;
+    Status fatal;  // a device error ends the loop; later tickets fail with it
     std::thread worker;
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
     double now() const
     {
         return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     }
+    void finish(uint64_t id, Result &r)
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            auto &it = items[id];
+            it.res = std::move(r);
+            it.t[2] = now();
+            it.done = true;
+        }
+        cv_done.notify_all();
+    }
 };
+
+static void fail_job(Server *s, uint64_t id, gfsm_status code)
+{
+    Result r;
+    r.status = code;
+    s->finish(id, r);
+}
 
 static void worker_main(Server *s)
 {
+    std::deque<std::unique_ptr<BatchJob>> pending;
+    auto done = [s](std::unique_ptr<BatchJob> job, Result &r) { s->finish(job->id, r); };
     for (;;) {
-        uint64_t id;
-        const HostGraph *q;
         {
             std::unique_lock<std::mutex> lk(s->mu);
-            s->cv.wait(lk, [s] { return s->stop || !s->queue.empty(); });
-            if (s->queue.empty()) return;
-            id = s->queue.front();
-            s->queue.pop_front();
-            q = s->items[id].q;
-            s->items[id].t[1] = s->now();
+            if (pending.empty() && s->eng->running() == 0)
+                s->cv_work.wait(lk, [s] { return s->stop || !s->incoming.empty(); });
+            while (!s->incoming.empty()) {
+                pending.push_back(std::move(s->incoming.front()));
+                s->incoming.pop_front();
+            }
+            if (s->stop && pending.empty() && s->eng->running() == 0) return;
         }
-        gfsm_match_opts o{};
-        o.flags = s->opts.flags;
-        Result r;
-        Status st = match_plain(*s->ctx, *s->data, *q, o, r);
-        if (!st.good()) r.status = st.code;
-        {
+        // Admission in arrival order while a slot and projected memory are free.
+        while (!pending.empty()) {
+            BatchJob &job = *pending.front();
+            const uint64_t id = job.id;
+            if (job.plan.q == 1 || (job.opts.flags & (GFSM_ROWS | GFSM_SORTED))) {
+                // No device steps (one vertex) or rows wanted: single-query path.
+                {
+                    std::lock_guard<std::mutex> lk(s->mu);
+                    s->items[id].t[1] = s->now();
+                }
+                Result r;
+                Status st = match_plain(*s->ctx, *s->data, *job.query, job.opts, r);
+                if (!st.good() && st.code != GFSM_ERR_CAP) r.status = st.code;
+                pending.pop_front();
+                s->finish(id, r);
+                continue;
+            }
+            Status st;
+            const double t = s->now();
+            if (!s->eng->try_admit(pending.front(), st)) {
+                if (!st.good()) {  // can never fit
+                    pending.pop_front();
+                    fail_job(s, id, st.code);
+                    continue;
+                }
+                break;
+            }
+            pending.pop_front();
             std::lock_guard<std::mutex> lk(s->mu);
-            auto &it = s->items[id];
-            it.res = std::move(r);
-            it.t[2] = s->now();
-            it.done = true;
+            s->items[id].t[1] = t;
         }
-        s->cv.notify_all();
+        if (s->eng->running()) {
+            Status st = s->eng->advance(done);
+            if (!st.good()) {
+                std::lock_guard<std::mutex> lk(s->mu);
+                s->fatal = st;
+                s->stop = true;
+                break;
+            }
+        }
     }
+    // Device error: fail everything still queued.
+    for (auto &j : pending) fail_job(s, j->id, GFSM_ERR_DEVICE);
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (auto &kv : s->items)
+        if (!kv.second.done) {
+            kv.second.res.status = GFSM_ERR_DEVICE;
+            kv.second.done = true;
+        }
+    s->cv_done.notify_all();
 }
 
 Status server_start(Ctx &ctx, const HostGraph &data, const gfsm_serve_opts &opts, Server *&out)
 {
-    Server *s = new Server;
+    out = nullptr;
+    std::unique_ptr<Server> s(new Server);
     s->ctx = &ctx;
     s->data = &data;
     s->opts = opts;
-    s->worker = std::thread(worker_main, s);
-    out = s;
+    BatchShape sh;
+    sh.slots = opts.max_inflight ? opts.max_inflight : ctx.tune.batch_slots;
+    sh.items = ctx.tune.batch_items;
+    sh.pool_words = ctx.tune.batch_pool_words;
+    sh.wgs_per_cu = ctx.tune.batch_wgs_per_cu;
+    sh.steps_per_replay = ctx.tune.batch_steps;
+    if (sh.slots > kMaxSlots) return Status::err(GFSM_ERR_ARG, "max_inflight must be at most 1024");
+    const uint64_t budget = opts.mem_budget ? opts.mem_budget : ctx.budget;
+    s->eng.reset(new BatchEngine(ctx, data));
+    Status st = s->eng->init(sh, budget);
+    if (!st.good()) return st;
+    // One query may take up to a sixteenth of the budget (its projection
+    // decides how much below that).
+    s->eng->set_max_region(std::max(s->eng->min_region(GFSM_MAXQ), budget / 16));
+    s->worker = std::thread(worker_main, s.get());
+    out = s.release();
     return Status::ok();
 }
 
-Status server_submit(Server &s, const HostGraph &query, uint64_t &ticket)
+Status server_submit(Server &s, const HostGraph &query, const gfsm_match_opts *opts, uint64_t &ticket)
 {
+    std::unique_ptr<BatchJob> job(new BatchJob);
+    job->query = &query;
+    if (opts) job->opts = *opts;
+    job->opts.flags |= s.opts.flags;
+    Status plan_st = (query.mode == s.data->mode) ? plan_query(query, job->plan)
+                                                   : Status::err(GFSM_ERR_ARG, "query and data graph use different modes");
     {
         std::lock_guard<std::mutex> lk(s.mu);
+        if (s.stop) return s.fatal.good() ? Status::err(GFSM_ERR_ARG, "server stopped") : s.fatal;
         ticket = s.next++;
+        job->id = ticket;
         auto &it = s.items[ticket];
-        it.q = &query;
         it.t[0] = s.now();
-        s.queue.push_back(ticket);
+        if (!plan_st.good()) {
+            it.res.status = plan_st.code;
+            it.t[1] = it.t[2] = it.t[0];
+            it.done = true;
+        } else {
+            s.incoming.push_back(std::move(job));
+        }
     }
-    s.cv.notify_all();
+    s.cv_work.notify_all();
     return Status::ok();
 }
 
@@ -91,7 +186,7 @@ Status server_take(Server &s, uint64_t ticket, bool wait, Result &out)
     if (it == s.items.end()) return Status::err(GFSM_ERR_ARG, "unknown ticket");
     if (!it->second.done) {
         if (!wait) return Status::err(GFSM_ERR_PENDING, "pending");
-        s.cv.wait(lk, [&] { return it->second.done; });
+        s.cv_done.wait(lk, [&] { return it->second.done; });
     }
     out = std::move(it->second.res);
     return Status::ok();
@@ -112,7 +207,7 @@ void server_stop(Server *s)
         std::lock_guard<std::mutex> lk(s->mu);
         s->stop = true;
     }
-    s->cv.notify_all();
+    s->cv_work.notify_all();
     if (s->worker.joinable()) s->worker.join();
     delete s;
 }

@@ -1,0 +1,137 @@
+// tune.cpp -- the autotuner (no reference counterpart: the reference
+// launches fixed shapes, HnGPU.cu:45-47).  It times a caller-chosen sample
+// under each candidate launch shape and keeps the fastest in the context:
+//   single-query path: candidates per thread x LDS staging pool x
+//                      persistent workgroups per CU,
+//   batch path:        the same three for the batch step kernel.
+// Every shape computes the same result; only time is compared.
+#include <cstdio>
+#include <cstdlib>
+
+#include "serve.h"
+
+namespace gfsm {
+
+namespace {
+
+struct Sample {
+    const HostGraph &data;
+    const std::vector<const HostGraph *> &queries;
+    const std::vector<uint32_t> &lo, &hi;
+};
+
+// Device milliseconds of the sample on the single-query path.
+Status time_plain(Ctx &ctx, const Sample &s, double &ms)
+{
+    ms = 0;
+    for (size_t i = 0; i < s.queries.size(); i++) {
+        gfsm_match_opts o{};
+        o.root_lo = s.lo[i];
+        o.root_hi = s.hi[i];
+        Result r;
+        Status st = match_plain(ctx, s.data, *s.queries[i], o, r);
+        if (!st.good()) return st;
+        ms += r.stats.device_ms;
+    }
+    return Status::ok();
+}
+
+Status time_batch(Ctx &ctx, const Sample &s, const BatchShape &shape, double &ms)
+{
+    std::vector<gfsm_match_opts> opts(s.queries.size());
+    for (size_t i = 0; i < opts.size(); i++) {
+        opts[i].root_lo = s.lo[i];
+        opts[i].root_hi = s.hi[i];
+    }
+    std::vector<Result> out;
+    ms = 0;
+    Status st = match_batch_shaped(ctx, s.data, s.queries, opts, out, shape, &ms);
+    if (!st.good()) return st;
+    for (auto &r : out)
+        if (r.status != GFSM_OK) return Status::err(r.status, "autotune sample query failed");
+    return Status::ok();
+}
+
+}  // namespace
+
+Status autotune(Ctx &ctx, const HostGraph &data, const std::vector<const HostGraph *> &queries,
+                const std::vector<uint32_t> &root_lo, const std::vector<uint32_t> &root_hi, uint32_t which)
+{
+    if (queries.empty()) return Status::err(GFSM_ERR_ARG, "empty autotune sample");
+    Sample s{data, queries, root_lo, root_hi};
+    const uint32_t item_set[] = {2, 4, 8};
+    const uint32_t pool_set[] = {4096, 8192, 14336};
+    const uint32_t wgs_set[] = {4, 8};
+    const bool verbose = std::getenv("GFSM_TUNE_VERBOSE") != nullptr;
+
+    if (which & GFSM_TUNE_PLAIN) {
+        const Tuning keep = ctx.tune;
+        double best = 1e300;
+        uint32_t bi = keep.items[0][1], bp = keep.pool_words, bw = keep.wgs_per_cu;
+        double warm;
+        Status st = time_plain(ctx, s, warm);  // first-launch costs out of the comparison
+        if (!st.good()) return st;
+        for (uint32_t it : item_set)
+            for (uint32_t pw : pool_set)
+                for (uint32_t wg : wgs_set) {
+                    if (expand_lds_bytes(it, pw) > kMaxDynLds) continue;
+                    for (auto &k : ctx.tune.items)
+                        for (auto &v : k) v = (uint8_t)it;
+                    ctx.tune.pool_words = pw;
+                    ctx.tune.wgs_per_cu = wg;
+                    double ms;
+                    st = time_plain(ctx, s, ms);
+                    if (!st.good()) {
+                        ctx.tune = keep;
+                        return st;
+                    }
+                    if (verbose) std::fprintf(stderr, "autotune plain items=%u pool=%u wgs=%u: %.3f ms\n", it, pw, wg, ms);
+                    if (ms < best) {
+                        best = ms;
+                        bi = it;
+                        bp = pw;
+                        bw = wg;
+                    }
+                }
+        for (auto &k : ctx.tune.items)
+            for (auto &v : k) v = (uint8_t)bi;
+        ctx.tune.pool_words = bp;
+        ctx.tune.wgs_per_cu = bw;
+    }
+    if (which & GFSM_TUNE_BATCH) {
+        BatchShape base;
+        base.slots = ctx.tune.batch_slots;
+        base.steps_per_replay = ctx.tune.batch_steps;
+        double best = 1e300;
+        BatchShape pick = base;
+        pick.items = ctx.tune.batch_items;
+        pick.pool_words = ctx.tune.batch_pool_words;
+        pick.wgs_per_cu = ctx.tune.batch_wgs_per_cu;
+        double warm;
+        Status st = time_batch(ctx, s, pick, warm);
+        if (!st.good()) return st;
+        for (uint32_t it : item_set)
+            for (uint32_t pw : pool_set)
+                for (uint32_t wg : wgs_set) {
+                    if (expand_lds_bytes(it, pw) > kMaxDynLds) continue;
+                    BatchShape sh = base;
+                    sh.items = it;
+                    sh.pool_words = pw;
+                    sh.wgs_per_cu = wg;
+                    double ms;
+                    st = time_batch(ctx, s, sh, ms);
+                    if (!st.good()) return st;
+                    if (verbose) std::fprintf(stderr, "autotune batch items=%u pool=%u wgs=%u: %.3f ms\n", it, pw, wg, ms);
+                    if (ms < best) {
+                        best = ms;
+                        pick = sh;
+                    }
+                }
+        ctx.tune.batch_items = pick.items;
+        ctx.tune.batch_pool_words = pick.pool_words;
+        ctx.tune.batch_wgs_per_cu = pick.wgs_per_cu;
+    }
+    return Status::ok();
+}
+
+}  // namespace gfsm

@@ -112,6 +112,46 @@ def config4_queries(data, n=256):
             for i in range(n)]
 
 
+# Root windows of the batched workload, measured once: for query i, the last
+# CONFIG4_ROOT_K[i] root candidates by id, the largest of n, n/4, n/16, ... (n =
+# the query's root candidates) whose count finishes within 5e7 candidates.
+# The queries in CONFIG4_CAPPED have no root whose subtree fits that bound;
+# they run over their last root under the work cap CONFIG4_CAP and report a
+# partial (capped) count.
+CONFIG4_ROOT_K = (
+    1, 208, 358, 5, 16, 163, 1, 62, 1, 440, 1, 3, 1, 1, 1, 6402, 1, 23, 3554, 1, 633, 6804, 10, 7, 62, 3, 1, 83,
+    16, 322, 268, 7, 1134, 12870, 6277, 2, 441, 6, 9, 4, 6552, 4, 12246, 7, 8575, 1, 6804, 16, 24, 16, 437, 7,
+    25, 25, 308, 1, 378, 1, 1, 327, 269, 1, 163, 2, 8, 1, 10, 4940, 11290, 1, 644, 251, 17, 85, 5, 1004, 1748,
+    19, 6, 40, 179, 7, 11887, 7, 1, 5, 505, 83, 27972, 119, 6263, 1, 859, 1, 88, 2, 300, 755, 30, 16035, 1004,
+    6, 40, 1, 704, 117, 132, 40, 14, 24, 116, 3554, 1639, 10433, 109, 7950, 1, 625, 111, 1, 2127, 1, 7637,
+    64142, 40, 1, 2620, 3, 1, 1, 117, 1, 768, 76, 652, 1748, 444, 7, 43, 6, 4, 15, 2561, 4, 332, 23955, 27, 7,
+    437, 10, 1, 269, 284, 4311, 4712, 17, 24, 4, 6, 192, 4452, 12, 3, 71, 15, 1, 1, 15, 3, 36, 391, 1, 8501,
+    10, 1, 1, 249, 4010, 40, 43, 340, 19, 163, 163, 107, 25, 6, 41766, 1, 1, 27, 9, 2620, 1, 2, 1, 15, 23, 63,
+    3, 83, 17246, 3989, 67, 1, 17, 1, 62, 1194, 62, 652, 1, 9881, 1, 22, 10413, 97, 62, 22, 2351, 2811, 62, 1,
+    2, 12, 861, 446, 64142, 329, 1, 6, 30, 73, 4, 119, 119, 27898, 3610, 82, 1, 5, 22, 2232, 11290, 176, 134,
+    2620, 46, 1, 82, 18151, 2, 8, 9268, 250, 1)
+CONFIG4_CAPPED = (
+    0, 8, 10, 12, 14, 26, 45, 58, 61, 65, 69, 91, 93, 103, 125, 128, 129, 131, 150, 166, 171, 174, 175, 188, 195,
+    213, 222, 229, 248, 255)
+CONFIG4_CAP = 50_000_000
+
+
+def config4_windows(data, queries, ks=CONFIG4_ROOT_K):
+    """Root-id windows [lo, hi) of the batched workload's queries."""
+    out = []
+    for q, k in zip(queries, ks):
+        roots = data.roots(q)
+        k = min(k, len(roots))
+        out.append((int(roots[-k]) if k else data.V, data.V))
+    return out
+
+
+def config4_caps(n=256):
+    """Per-query work caps (0 = none) of the batched workload."""
+    capped = set(CONFIG4_CAPPED)
+    return [CONFIG4_CAP if i in capped else 0 for i in range(n)]
+
+
 def config5_data(scale=23):
     return Graph.rmat(scale, 16, *RMAT_ABC, nvlabels=64, zipf_s=1.0, nelabels=4, seed=5000, mode=MODE_VLABEL)
 

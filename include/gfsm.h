@@ -162,10 +162,16 @@ void        gfsm_result_free(gfsm_result *r);
  * the row (column order = original query id). */
 void        gfsm_row_hash(const uint32_t *row, uint32_t width, uint64_t h[2]);
 
-/* ---- batching: n queries share each depth step; one persistent launch per
- * depth with a device work queue of (query, tile) items ------------------ */
+/* ---- batching (no counterpart): n queries share every step.  Each step is
+ * one scheduler launch plus one persistent step launch whose device work
+ * queue holds the (query, tile) items of every query in flight; runs of steps
+ * are captured in a hipGraph and replayed.  opts holds nopts entries: 1 (the
+ * same options for every query) or n (one per query).  Count and digest modes
+ * run batched; queries asking for rows run on the single-query path.
+ * results[i] is always set (free each); its gfsm_result_status is the query's
+ * own status.  The call fails only for invalid arguments or device errors. */
 gfsm_status gfsm_match_batch(gfsm_ctx *ctx, const gfsm_graph *data, const gfsm_graph *const *queries,
-                             uint32_t n, const gfsm_match_opts *opts, gfsm_result **results);
+                             uint32_t n, const gfsm_match_opts *opts, uint32_t nopts, gfsm_result **results);
 
 /* ---- serving: resident data graph, admission by projected memory,
  * continuous batching -------------------------------------------------------- */
@@ -177,7 +183,11 @@ typedef struct gfsm_serve_opts {
 
 gfsm_status gfsm_server_start(gfsm_ctx *ctx, const gfsm_graph *data, const gfsm_serve_opts *opts,
                               gfsm_server **srv);
-gfsm_status gfsm_server_submit(gfsm_server *srv, const gfsm_graph *query, uint64_t *ticket);
+/* opts (may be NULL): this query's flags (added to the server's), root
+ * window and work cap; its mem_budget and tile are ignored (the server sizes
+ * each query's memory by projection). */
+gfsm_status gfsm_server_submit(gfsm_server *srv, const gfsm_graph *query, const gfsm_match_opts *opts,
+                               uint64_t *ticket);
 /* GFSM_ERR_PENDING while running; on GFSM_OK *r is owned by the caller. */
 gfsm_status gfsm_server_poll(gfsm_server *srv, uint64_t ticket, gfsm_result **r);
 gfsm_status gfsm_server_wait(gfsm_server *srv, uint64_t ticket, gfsm_result **r);
@@ -191,6 +201,37 @@ void        gfsm_server_stop(gfsm_server *srv);
  * totals[0] = number of non-zero values, totals[1] = sum.  Host arrays. */
 gfsm_status gfsm_selftest_scan(gfsm_ctx *ctx, const uint32_t *vals, uint64_t n, uint64_t *prefix,
                                uint32_t *idx, uint64_t totals[2]);
+
+/* ---- tile shapes and the autotuner (no counterpart) ------------------------
+ * The kernels' launch shapes, per context.  items[kind][d]: candidates per
+ * thread (1, 2, 4 or 8; a tile is 256 x items candidates) of the single-query
+ * expand kernel for kind 0 = inner depth, 1 = last depth counting, 2 = last
+ * depth writing rows, at depth d.  pool_words: LDS words per workgroup that
+ * stage the tile's frontier nodes (checked segments, placed vertices).
+ * wgs_per_cu: persistent workgroups per compute unit.  batch_*: the same for
+ * the batch step kernel; batch_slots: queries in flight in one batch. */
+typedef struct gfsm_tuning {
+    uint32_t items[3][GFSM_MAX_Q];
+    uint32_t pool_words;
+    uint32_t wgs_per_cu;
+    uint32_t batch_items;
+    uint32_t batch_pool_words;
+    uint32_t batch_wgs_per_cu;
+    uint32_t batch_slots;
+} gfsm_tuning;
+gfsm_status gfsm_tuning_get(gfsm_ctx *ctx, gfsm_tuning *t);
+gfsm_status gfsm_tuning_set(gfsm_ctx *ctx, const gfsm_tuning *t);
+
+/* Autotuner: runs the sample -- n queries, each over its root window
+ * [root_lo[i], root_hi[i]) (NULL arrays = whole range), count mode -- under
+ * each candidate shape, keeps the fastest in ctx and returns it in *t.
+ * which: 1 = single-query shapes, 2 = batch shapes, 3 = both.  The sample
+ * should take tens of milliseconds per shape. */
+#define GFSM_TUNE_PLAIN 1u
+#define GFSM_TUNE_BATCH 2u
+gfsm_status gfsm_autotune(gfsm_ctx *ctx, const gfsm_graph *data, const gfsm_graph *const *queries,
+                          const uint32_t *root_lo, const uint32_t *root_hi, uint32_t n, uint32_t which,
+                          gfsm_tuning *t);
 
 /* ---- seeded workload generators (undirected graphs as symmetric entry lists) */
 #define GFSM_WALK_DENSE 0u   /* keep every induced edge among the walked vertices */

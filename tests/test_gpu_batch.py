@@ -1,0 +1,167 @@
+"""GPU parity of the batched path (device scheduler + persistent step
+kernel, hipGraph replays) and of the serving loop, against the CPU oracle and
+the single-query path."""
+import random
+import time
+
+import numpy as np
+import pytest
+
+import gfsm_amd as G
+from helpers import oracle_solve
+
+pytestmark = pytest.mark.gpu
+
+
+def _queries(data, sizes, seed):
+    qs = []
+    for i, s in enumerate(sizes):
+        kind = G.WALK_DENSE if i % 2 == 0 else G.WALK_SPARSE
+        qs.append(G.Graph.walk_query(data, s, kind=kind, seed=seed + i))
+    return qs
+
+
+@pytest.fixture(scope="module")
+def er_vlabel():
+    return G.Graph.er(1500, 7.0, nvlabels=3, nelabels=2, seed=501, mode=G.MODE_VLABEL)
+
+
+def test_batch_equals_oracle(ctx, er_vlabel):
+    data = er_vlabel
+    qs = _queries(data, [2, 3, 4, 5, 6, 7, 3, 4, 5, 6, 2, 8], seed=900)
+    ms = ctx.match_batch(data, qs, digest=True)
+    for q, m in zip(qs, ms):
+        ref = oracle_solve(data, q, G.MODE_VLABEL, rows=False)
+        assert m.status == 0
+        assert (m.count, m.digest) == (ref.count, ref.digest)
+
+
+def test_batch_ref_mode_and_windows(ctx):
+    data = G.Graph.rmat(11, 8, nvlabels=1, zipf_s=1.0, nelabels=3, seed=77, mode=G.MODE_REF)
+    qs = _queries(data, [3, 4, 5, 4, 3, 5, 6, 4], seed=70)
+    windows = [(0, 0), (0, 500), (100, 1500), (0, 0), (1000, 2048), (7, 9), (0, 300), (2000, 2001)]
+    ms = ctx.match_batch(data, qs, windows=windows, digest=True)
+    for q, (lo, hi), m in zip(qs, windows, ms):
+        ref = ctx.match(data, q, root_lo=lo, root_hi=hi, digest=True)
+        assert (m.count, m.digest) == (ref.count, ref.digest), (lo, hi)
+
+
+def test_batch_few_slots_continuous_admission(ctx, er_vlabel):
+    """More queries than slots: finished slots are refilled between replays."""
+    data = er_vlabel
+    qs = _queries(data, [3, 5, 4, 6, 2, 7, 5, 3, 4, 6, 5, 4, 3, 6, 5, 7], seed=1300)
+    ref = [ctx.match(data, q, digest=True) for q in qs]
+    saved = ctx.tuning()
+    try:
+        for slots in (1, 3, 5):
+            ctx.set_tuning(batch_slots=slots)
+            ms = ctx.match_batch(data, qs, digest=True)
+            assert [(m.count, m.digest) for m in ms] == [(r.count, r.digest) for r in ref], slots
+    finally:
+        ctx.set_tuning(batch_slots=saved["batch_slots"])
+
+
+@pytest.mark.parametrize("items,pool", [(1, 1024), (2, 4096), (4, 8192), (8, 14336), (8, 64)])
+def test_batch_shapes(ctx, er_vlabel, items, pool):
+    """Every tile shape and staging pool (a tiny pool forces unstaged checks)."""
+    data = er_vlabel
+    qs = _queries(data, [4, 5, 6, 3, 7], seed=2100)
+    ref = [ctx.match(data, q, digest=True) for q in qs]
+    saved = ctx.tuning()
+    try:
+        ctx.set_tuning(batch_items=items, batch_pool_words=pool)
+        ms = ctx.match_batch(data, qs, digest=True)
+        assert [(m.count, m.digest) for m in ms] == [(r.count, r.digest) for r in ref]
+    finally:
+        ctx.set_tuning(batch_items=saved["batch_items"], batch_pool_words=saved["batch_pool_words"])
+
+
+def test_batch_device_chunking(ctx):
+    """A tight arena: levels overflow their regions and the device scheduler
+    halves and re-estimates chunks; counts are unchanged."""
+    data = G.Graph.grid(60, 60, del_frac=0.05, diag_frac=0.02, seed=3, mode=G.MODE_REF)
+    qs = _queries(data, [6, 7, 8, 6, 7], seed=4400)
+    ref = [ctx.match(data, q, digest=True) for q in qs]
+    ms = ctx.match_batch(data, qs, digest=True, mem_budget=48 << 20)
+    assert [(m.count, m.digest) for m in ms] == [(r.count, r.digest) for r in ref]
+    assert any(m.stats["overflows"] > 0 or m.stats["launches"] > 2 * m.stats["q"] for m in ms)
+
+
+def test_batch_work_cap_and_mixed_flags(ctx, er_vlabel):
+    data = er_vlabel
+    qs = _queries(data, [5, 6], seed=5100)
+    full = [ctx.match(data, q) for q in qs]
+    ms = ctx.match_batch(data, qs, max_cand=1000)
+    for m, f in zip(ms, full):
+        if sum(f.stats["cand"]) > 1000:
+            assert m.status == 5 and m.count <= f.count
+    # rows requested: served by the single-query path inside the batch call
+    ms = ctx.match_batch(data, qs, rows=True, sorted=True)
+    for q, m in zip(qs, ms):
+        ref = oracle_solve(data, q, G.MODE_VLABEL)
+        assert np.array_equal(m.rows, ref.rows)
+
+
+def test_batch_invalid_query_status(ctx, er_vlabel, tmp_path):
+    p = tmp_path / "disc.g"  # two components: rejected like the reference (exit 1)
+    p.write_text("t # 0\n4 2 1 1\nv 0 1\nv 1 1\nv 2 1\nv 3 1\ne 0 1 1\ne 1 0 1\nt # -1")
+    bad = G.Graph.load(p, mode=G.MODE_VLABEL)
+    good = G.Graph.walk_query(er_vlabel, 4, seed=5)
+    ms = ctx.match_batch(er_vlabel, [good, bad, good])
+    assert ms[0].status == 0 and ms[2].status == 0 and ms[1].status == 1
+    assert ms[0].count == ms[2].count == ctx.match(er_vlabel, good).count
+
+
+def test_server_alone_vs_mixed(ctx, er_vlabel):
+    """Each served result equals the same query run alone, while queries of
+    different sizes (some over root windows) arrive during others' work."""
+    data = er_vlabel
+    rng = random.Random(5)
+    qs = _queries(data, [rng.randint(2, 7) for _ in range(40)], seed=6000)
+    wins = [(0, 0) if i % 3 else (rng.randint(0, 700), rng.randint(800, 1500)) for i in range(len(qs))]
+    alone = [ctx.match(data, q, digest=True, root_lo=lo, root_hi=hi) for q, (lo, hi) in zip(qs, wins)]
+    srv = G.Server(ctx, data, max_inflight=6, digest=True)
+    try:
+        tickets = []
+        for q, (lo, hi) in zip(qs, wins):
+            tickets.append(srv.submit(q, root_lo=lo, root_hi=hi))
+            time.sleep(rng.expovariate(4000.0))
+        got = [srv.wait(t) for t in tickets]
+        times = [srv.times(t) for t in tickets]
+    finally:
+        srv.stop()
+    assert [(m.count, m.digest) for m in got] == [(a.count, a.digest) for a in alone]
+    for t0, t1, t2 in times:
+        assert t0 <= t1 <= t2
+
+
+def test_server_statuses(ctx, er_vlabel, tmp_path):
+    p = tmp_path / "disc.g"
+    p.write_text("t # 0\n4 2 1 1\nv 0 1\nv 1 1\nv 2 1\nv 3 1\ne 0 1 1\ne 1 0 1\nt # -1")
+    bad = G.Graph.load(p, mode=G.MODE_VLABEL)
+    one = G.Graph.walk_query(er_vlabel, 1, seed=3)
+    good = G.Graph.walk_query(er_vlabel, 4, seed=5)
+    srv = G.Server(ctx, er_vlabel, max_inflight=2)
+    try:
+        tb, t1, tg = srv.submit(bad), srv.submit(one), srv.submit(good)
+        mb, m1, mg = srv.wait(tb), srv.wait(t1), srv.wait(tg)
+    finally:
+        srv.stop()
+    assert mb.status == 1
+    assert m1.status == 0 and m1.count == ctx.match(er_vlabel, one).count
+    assert mg.status == 0 and mg.count == ctx.match(er_vlabel, good).count
+
+
+def test_autotune_keeps_results(ctx, er_vlabel):
+    data = er_vlabel
+    qs = _queries(data, [4, 5, 6], seed=7100)
+    ref = [ctx.match(data, q, digest=True) for q in qs]
+    saved = ctx.tuning()
+    try:
+        t = ctx.autotune(data, qs)
+        assert t["batch_items"] in (2, 4, 8) and t["items"][0][1] in (2, 4, 8)
+        ms = ctx.match_batch(data, qs, digest=True)
+        assert [(m.count, m.digest) for m in ms] == [(r.count, r.digest) for r in ref]
+        assert [(ctx.match(data, q, digest=True).count) for q in qs] == [r.count for r in ref]
+    finally:
+        ctx.set_tuning(**{k: v for k, v in saved.items()})

@@ -1,19 +1,27 @@
-"""Benchmark: plain single-query matching on the power-law workload.
+"""Benchmark: the matcher on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+                    [--sections plain,batch,serve] [--serve-queries N]
 
-A step is one pass of the plain matcher over the 20 queries of the power-law
-workload (R-MAT scale 18, edge factor 8, 16 vertex labels Zipf(1.0), 1 edge
-label, labelled mode), each restricted to its fixed root window (the last K
-root candidates by id; see gfsm_amd/workloads.py), counting embeddings on the
-device.  Data graph and queries are generated from fixed seeds and the data
-graph is resident in HBM before the timed region.
-
-Prints one JSON line: embeddings/s over the timed steps, the roofline of the
+plain (the headline `value`): a step is one pass of the single-query matcher
+over the 20 queries of the power-law workload (R-MAT scale 18, edge factor 8,
+16 vertex labels Zipf(1.0), 1 edge label, labelled mode), each restricted to
+its fixed root window (the last K root candidates by id; see
+gfsm_amd/workloads.py), counting embeddings on the device.  The data graph is
+resident in HBM before the timed region.  Also reported: the roofline of the
 dominant kernel (the expand kernel: algorithmic bytes per launch / mean launch
-time, timed with HIP events on the matcher's stream inside the timed region),
-and the CPU oracle timed on a bounded sample of the same workload on this
-host.
+time, timed with HIP events on the matcher's stream inside the timed region)
+and the CPU oracle on a bounded sample of the same workload on this host.
+
+batch: the 256 queries of the batched workload (R-MAT scale 22) over their
+root windows, once one at a time on the single-query path and once in one
+batched call (autotuned shapes, captured steps); queries/s of both.
+
+serve: the served stream (R-MAT scale 23, edge factor 16) -- its queries run
+one at a time on the single-query path (serial), then through the server
+with all queries submitted at once (saturated throughput), then as an
+open-loop Poisson stream; queries/s and p50/p99 latency of each, and whether
+every uncapped query's count and digest are the same served and serial.
 """
 import argparse
 import json
@@ -34,6 +42,9 @@ def parse():
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline")
     p.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample budget")
+    p.add_argument("--sections", default="plain,batch,serve",
+                   help="comma-separated sections to run (plain is the headline value)")
+    p.add_argument("--serve-queries", type=int, default=10000, help="length of the served stream")
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                    help="only run an elementwise int32 kernel that reads and writes MIB MiB, 5 times "
                         "(scales the HBM byte counters of a profiler run)")
@@ -93,6 +104,137 @@ def cpu_baseline(data, queries, windows, budget_s):
                       f"({emb} embeddings in {spent:.1f} s, OpenMP)"}
 
 
+def pct(xs, p):
+    import numpy as np
+
+    return float(np.percentile(np.asarray(xs, dtype=float), p)) if len(xs) else 0.0
+
+
+def batch_section():
+    """The batched workload: serial single-query pass vs one batched call."""
+    import gfsm_amd as G
+    import gfsm_amd.workloads as W
+
+    data = W.config4_data()
+    qs = W.config4_queries(data)
+    wins = W.config4_windows(data, qs)
+    caps = W.config4_caps(len(qs))
+    mc = [c for c, _ in caps]
+    ml = [l for _, l in caps]
+    ctx = G.Context(0, mem_budget=64 << 30)
+    ctx.upload(data)
+    # autotune the batch shapes on the uncapped queries among the first 64
+    sample = [i for i in range(64) if ml[i] == 0]
+    tuned = ctx.autotune(data, [qs[i] for i in sample], windows=[wins[i] for i in sample], which=G.TUNE_BATCH)
+    ctx.match_batch(data, qs[:16], windows=wins[:16], caps=mc[:16], depth_caps=ml[:16])  # engine set-up
+    for i in range(16):  # single-query arena set-up, likewise outside the timing
+        ctx.match(data, qs[i], root_lo=wins[i][0], root_hi=wins[i][1], max_cand=mc[i], max_depth_cand=ml[i],
+                  allow_partial=True)
+    t = time.perf_counter()
+    ser = [ctx.match(data, q, root_lo=lo, root_hi=hi, max_cand=c, max_depth_cand=l, allow_partial=True, digest=True)
+           for q, (lo, hi), c, l in zip(qs, wins, mc, ml)]
+    t_ser = time.perf_counter() - t
+    t = time.perf_counter()
+    bat = ctx.match_batch(data, qs, windows=wins, caps=mc, depth_caps=ml, digest=True)
+    t_bat = time.perf_counter() - t
+    equal = sum((b.count, b.digest, b.status) == (s_.count, s_.digest, s_.status) for b, s_ in zip(bat, ser))
+    emb = sum(m.count for m in bat)
+    ctx.close()
+    return {
+        "workload": "batched: R-MAT scale 22, edge factor 8, 32 vertex labels Zipf(1.2), 4 edge labels; "
+                    "256 random-walk queries of 4-16 vertices over fixed root windows; 30 of them capped "
+                    "deterministically at 2^22 candidates per depth",
+        "queries": len(qs),
+        "batched_queries_per_s": len(qs) / t_bat,
+        "batched_embeddings_per_s": emb / t_bat,
+        "batched_ms": t_bat * 1e3,
+        "serial_queries_per_s": len(qs) / t_ser,
+        "serial_ms": t_ser * 1e3,
+        "speedup_vs_serial": t_ser / t_bat,
+        "equal_to_serial": f"{equal}/{len(qs)}",
+        "capped": sum(m.status == 5 for m in bat),
+        "tuned_shape": {k: tuned[k] for k in ("batch_items", "batch_pool_words", "batch_wgs_per_cu", "batch_slots")},
+    }
+
+
+def serve_section(n):
+    """The served stream: serial single-query pass, saturated server, and an
+    open-loop Poisson stream."""
+    import numpy as np
+
+    import gfsm_amd as G
+    import gfsm_amd.workloads as W
+
+    data = W.config5_data()
+    qs = W.config5_queries(data, n)
+    wins = W.config5_windows(data, qs)
+    cap, last = W.CONFIG5_CAP, W.CONFIG5_DEPTH_CAP
+    ctx = G.Context(0, mem_budget=16 << 30)
+    ctx.upload(data)
+    for q, (lo, hi) in list(zip(qs, wins))[:32]:  # warm the single-query path
+        ctx.match(data, q, root_lo=lo, root_hi=hi, max_cand=cap, max_depth_cand=last, allow_partial=True)
+    svc, ser = [], []
+    t = time.perf_counter()
+    for q, (lo, hi) in zip(qs, wins):
+        t1 = time.perf_counter()
+        ser.append(ctx.match(data, q, root_lo=lo, root_hi=hi, max_cand=cap, max_depth_cand=last, allow_partial=True,
+                             digest=True))
+        svc.append(time.perf_counter() - t1)
+    t_ser = time.perf_counter() - t
+    budget = 200 * 10**9
+    srv = G.Server(ctx, data, mem_budget=budget, max_inflight=256, digest=True)
+    try:
+        t = time.perf_counter()
+        tk = [srv.submit(q, root_lo=lo, root_hi=hi, max_cand=cap, max_depth_cand=last) for q, (lo, hi) in zip(qs, wins)]
+        sat = [srv.wait(x) for x in tk]
+        t_sat = time.perf_counter() - t
+        rate = 0.8 * n / t_sat
+        arr = W.poisson_arrivals(n, rate)
+        tk = []
+        t0 = time.perf_counter()
+        for a, q, (lo, hi) in zip(arr, qs, wins):
+            d = a - (time.perf_counter() - t0)
+            if d > 0:
+                time.sleep(d)
+            tk.append(srv.submit(q, root_lo=lo, root_hi=hi, max_cand=cap, max_depth_cand=last))
+        ol = [srv.wait(x) for x in tk]
+        times = [srv.times(x) for x in tk]
+    finally:
+        srv.stop()
+    lat = [t2 - t0_ for t0_, t1, t2 in times]
+    # the same arrivals served one at a time (FIFO) with the measured serial times
+    done, slat = 0.0, []
+    for a, s_ in zip(arr, svc):
+        done = max(done, a) + s_
+        slat.append(done - a)
+    capped = [i for i in range(n) if ser[i].status == 5]
+    key = lambda m: (m.count, m.digest, m.status)
+    eq_sat = sum(key(a) == key(b) for a, b in zip(sat, ser))
+    eq_ol = sum(key(a) == key(b) for a, b in zip(ol, ser))
+    ctx.close()
+    return {
+        "workload": f"served: R-MAT scale 23, edge factor 16, 64 vertex labels Zipf(1.0), 4 edge labels, resident; "
+                    f"{n} random-walk queries of 4-24 vertices (4 + geometric, mean 8), each over the highest-id roots "
+                    f"with <= {W.CONFIG5_FIRST} depth-1 candidates, capped deterministically at {last} candidates "
+                    f"per depth; "
+                    f"admission by projected memory against a {budget // 10**9} GB budget, 256 slots",
+        "queries": n,
+        "served_queries_per_s": n / t_sat,
+        "serial_queries_per_s": n / t_ser,
+        "served_vs_serial": t_ser / t_sat,
+        "open_loop_rate_per_s": rate,
+        "open_loop_p50_ms": pct(lat, 50) * 1e3,
+        "open_loop_p99_ms": pct(lat, 99) * 1e3,
+        "serial_same_arrivals_p50_ms": pct(slat, 50) * 1e3,
+        "serial_same_arrivals_p99_ms": pct(slat, 99) * 1e3,
+        "serial_service_p50_ms": pct(svc, 50) * 1e3,
+        "capped": len(capped),
+        "equal_saturated": f"{eq_sat}/{n}",
+        "equal_open_loop": f"{eq_ol}/{n}",
+        "embeddings": int(sum(m.count for m in sat)),
+    }
+
+
 def main():
     args = parse()
     if args.calibrate_copy:
@@ -105,6 +247,7 @@ def main():
     import gfsm_amd as G
     import gfsm_amd.workloads as W
 
+    sections = set(args.sections.split(","))
     data = W.config2_data()
     queries = W.config2_queries(data)
     windows = W.config2_windows(data, queries)
@@ -170,10 +313,15 @@ def main():
             "mean_launch_ms": mean_ms,
         },
     }
+    ctx.close()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(data, queries, windows, args.cpu_seconds)
     else:
         line["cpu_baseline"] = None
+    if "batch" in sections:
+        line["batched"] = batch_section()
+    if "serve" in sections:
+        line["served"] = serve_section(args.serve_queries)
     print(json.dumps(line))
 
 

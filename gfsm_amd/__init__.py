@@ -29,6 +29,8 @@ MODE_VLABEL = 1
 ROWS = 1
 SORTED = 2
 DIGEST = 4
+TUNE_PLAIN = 1
+TUNE_BATCH = 2
 PROFILE = 8
 WALK_DENSE = 0
 WALK_SPARSE = 1
@@ -45,7 +47,7 @@ class GfsmError(RuntimeError):
 class MatchOpts(C.Structure):
     _fields_ = [("flags", C.c_uint32), ("root_lo", C.c_uint32), ("root_hi", C.c_uint32),
                 ("tile", C.c_uint32), ("max_rows", C.c_uint64), ("mem_budget", C.c_uint64),
-                ("max_cand", C.c_uint64)]
+                ("max_cand", C.c_uint64), ("max_depth_cand", C.c_uint64)]
 
 
 class Stats(C.Structure):
@@ -114,6 +116,7 @@ def lib():
         "gfsm_graph_free": [vp],
         "gfsm_plan_query": [vp, P(u32), P(u32), pu32, pu32, pu32],
         "gfsm_query_roots": [vp, vp, pu32, P(u64)],
+        "gfsm_query_window": [vp, vp, u64, pu32, pu32, P(u64)],
         "gfsm_match": [vp, vp, vp, P(MatchOpts), P(vp)],
         "gfsm_result_count": [vp, P(u64)],
         "gfsm_result_rows": [vp, P(pu32), P(u32), P(u64)],
@@ -269,6 +272,14 @@ class Graph:
                                       C.byref(n)), "roots")
         return ids[: n.value]
 
+    def window(self, query, max_first):
+        """(root_lo, root_hi, first): the highest-id roots of `query` whose
+        depth-1 candidates total at most max_first (at least one root)."""
+        lo, hi, f = C.c_uint32(), C.c_uint32(), C.c_uint64()
+        _check(lib().gfsm_query_window(self._h, query._h, max_first, C.byref(lo), C.byref(hi), C.byref(f)),
+               "window")
+        return lo.value, hi.value, f.value
+
     def plan(self):
         """Reference-layout plan: (col, header[3q], requirements[2*maxval*q])."""
         q, mv = C.c_uint32(), C.c_uint32()
@@ -314,10 +325,10 @@ class Match:
 
 
 def _opts(rows=False, sorted=False, digest=False, profile=False, root_lo=0, root_hi=0, max_rows=0,
-          mem_budget=0, max_cand=0, tile=0):
+          mem_budget=0, max_cand=0, tile=0, max_depth_cand=0):
     flags = (ROWS if rows else 0) | (SORTED if sorted else 0) | (DIGEST if digest else 0) | \
         (PROFILE if profile else 0)
-    return MatchOpts(flags, root_lo, root_hi, tile, max_rows, mem_budget, max_cand)
+    return MatchOpts(flags, root_lo, root_hi, tile, max_rows, mem_budget, max_cand, max_depth_cand)
 
 
 class Context:
@@ -369,19 +380,20 @@ class Context:
         _check(rc, "match")
         return Match(r)
 
-    def match_batch(self, data: Graph, queries, windows=None, caps=None, **kw):
+    def match_batch(self, data: Graph, queries, windows=None, caps=None, depth_caps=None, **kw):
         """Batched matching: every query of the list in flight together.
-        windows: optional per-query (root_lo, root_hi); caps: optional
-        per-query max_cand.  Returns one Match per query; check each Match's
-        status (5 = capped, partial count)."""
+        windows: optional per-query (root_lo, root_hi); caps / depth_caps:
+        optional per-query max_cand / max_depth_cand.  Returns one Match per query;
+        check each Match's status (5 = capped, partial count)."""
         n = len(queries)
-        if windows is None and caps is None:
+        if windows is None and caps is None and depth_caps is None:
             os_ = (MatchOpts * 1)(_opts(**kw))
         else:
             wins = windows if windows is not None else [(0, 0)] * n
             cps = caps if caps is not None else [kw.pop("max_cand", 0)] * n
-            os_ = (MatchOpts * n)(*[_opts(root_lo=lo, root_hi=hi, max_cand=c, **kw)
-                                    for (lo, hi), c in zip(wins, cps)])
+            lcs = depth_caps if depth_caps is not None else [kw.pop("max_depth_cand", 0)] * n
+            os_ = (MatchOpts * n)(*[_opts(root_lo=lo, root_hi=hi, max_cand=c, max_depth_cand=l, **kw)
+                                    for (lo, hi), c, l in zip(wins, cps, lcs)])
         hs = (C.c_void_p * n)(*[q._h.value for q in queries])
         rs = (C.c_void_p * n)()
         _check(lib().gfsm_match_batch(self._h, data._h, hs, n, os_, len(os_), rs), "match_batch")
@@ -431,7 +443,7 @@ class Server:
         self._queries = {}
 
     def submit(self, query: Graph, **kw) -> int:
-        """Queue a query; kw: root_lo, root_hi, max_cand, digest (see Context.match)."""
+        """Queue a query; kw: root_lo, root_hi, max_cand, max_depth_cand, digest (see Context.match)."""
         t = C.c_uint64()
         o = _opts(**kw)
         _check(lib().gfsm_server_submit(self._h, query._h, C.byref(o), C.byref(t)), "submit")

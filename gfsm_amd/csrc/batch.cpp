@@ -198,6 +198,9 @@ BatchEngine::~BatchEngine()
     if (stage_h_) (void)hipHostFree(stage_h_);
     if (report_h_) (void)hipHostFree(report_h_);
     if (step_cand_h_) (void)hipHostFree(step_cand_h_);
+    if (n_admit_h_) (void)hipHostFree(n_admit_h_);
+    if (admit_h_) (void)hipHostFree(admit_h_);
+    (void)hipFree(admit_d_);
 }
 
 Status BatchEngine::init(const BatchShape &shape, uint64_t arena_bytes, char *arena)
@@ -230,6 +233,9 @@ Status BatchEngine::init(const BatchShape &shape, uint64_t arena_bytes, char *ar
     HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&stage_h_), sizeof(BatchSlot) * n, hipHostMallocDefault));
     HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&report_h_), sizeof(SlotReport) * n, hipHostMallocDefault));
     HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&step_cand_h_), sizeof(uint64_t), hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&n_admit_h_), sizeof(uint32_t), hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&admit_h_), sizeof(AdmitEntry) * n, hipHostMallocDefault));
+    HIP_TRY(hipMalloc(&admit_d_, sizeof(AdmitEntry) * n));
     std::memset(stage_h_, 0, sizeof(BatchSlot) * n);
     for (uint32_t s = 0; s < n; s++) {
         stage_h_[s].state = SLOT_FREE;
@@ -257,6 +263,8 @@ Status BatchEngine::init(const BatchShape &shape, uint64_t arena_bytes, char *ar
 
     cfg_ = BatchCfg{};
     cfg_.slots = slots_d_;
+    cfg_.plans = plans_d_;
+    cfg_.admit = admit_d_;
     cfg_.report = report_d_;
     cfg_.tile_start = tile_start_d_;
     cfg_.ctl = ctl_d_;
@@ -359,8 +367,11 @@ bool BatchEngine::try_admit(std::unique_ptr<BatchJob> &job, Status &st)
     job->region_bytes = want;
     job->t_admit = now_ms();
 
-    BatchSlot &s = stage_h_[slot];
-    std::memset(&s, 0, sizeof(BatchSlot));
+    AdmitEntry &ae = admit_h_[n_admit_];
+    std::memset(&ae, 0, sizeof(AdmitEntry));
+    ae.slot = slot;
+    ae.plan = plan.dev;
+    BatchSlot &s = ae.s;
     s.plan = plans_d_ + slot;
     s.root_vtx = dg_->rel_vtx + rlo;
     s.root_beg = dg_->rel_beg + rlo;
@@ -368,19 +379,12 @@ bool BatchEngine::try_admit(std::unique_ptr<BatchJob> &job, Status &st)
     s.nroots = rhi - rlo;
     carve(s, plan, arena_ + off, want);
     s.max_cand = job->opts.max_cand;
+    s.max_depth_cand = job->opts.max_depth_cand;
     s.q = plan.q;
     s.digest = (job->opts.flags & GFSM_DIGEST) ? 1u : 0u;
     s.state = SLOT_NEW;
     s.step = STEP_NONE;
-    hipError_t e = hipMemcpyAsync(plans_d_ + slot, &plan.dev, sizeof(QueryPlanDev), hipMemcpyHostToDevice,
-                                  lane_.stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(slots_d_ + slot, &s, sizeof(BatchSlot), hipMemcpyHostToDevice, lane_.stream);
-    if (e != hipSuccess) {
-        regions_.give(off, want);
-        st = hip_status(e, "batch admission");
-        return false;
-    }
+    n_admit_++;  // installed by the next replay's first scheduler launch
     jobs_[slot] = std::move(job);
     running_++;
     return true;
@@ -404,6 +408,14 @@ Status BatchEngine::replay()
         HIP_TRY(launch_ws_clear(lane_.ws, lane_.stream));
         since_clear_ = 0;
     }
+    if (n_admit_) {
+        // One copy of the queued admissions; the scheduler installs them.
+        *n_admit_h_ = n_admit_;
+        HIP_TRY(hipMemcpyAsync(admit_d_, admit_h_, sizeof(AdmitEntry) * n_admit_, hipMemcpyHostToDevice,
+                               lane_.stream));
+        HIP_TRY(hipMemcpyAsync(&ctl_d_->n_admit, n_admit_h_, sizeof(uint32_t), hipMemcpyHostToDevice,
+                               lane_.stream));
+    }
     HIP_TRY(hipEventRecord(ev_[0], lane_.stream));
     if (exec_) {
         HIP_TRY(hipGraphLaunch(exec_, lane_.stream));
@@ -419,6 +431,7 @@ Status BatchEngine::replay()
     HIP_TRY(hipMemcpyAsync(report_h_, report_d_, sizeof(SlotReport) * shape_.slots, hipMemcpyDeviceToHost,
                            lane_.stream));
     HIP_TRY(hipStreamSynchronize(lane_.stream));
+    n_admit_ = 0;
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
     device_ms_ += ms;
@@ -441,8 +454,9 @@ Status BatchEngine::harvest(const std::function<void(std::unique_ptr<BatchJob>, 
         if (st == SLOT_DONE || st == SLOT_CAP || st == SLOT_ERROR) fin.push_back(s);
     }
     if (fin.empty()) return Status::ok();
-    for (uint32_t s : fin)
-        HIP_TRY(hipMemcpyAsync(stage_h_ + s, slots_d_ + s, sizeof(BatchSlot), hipMemcpyDeviceToHost, lane_.stream));
+    // One copy of the slot table (a few MB at most) rather than one per slot.
+    HIP_TRY(hipMemcpyAsync(stage_h_, slots_d_, sizeof(BatchSlot) * shape_.slots, hipMemcpyDeviceToHost,
+                           lane_.stream));
     HIP_TRY(hipStreamSynchronize(lane_.stream));
     const double t = now_ms();
     for (uint32_t s : fin) {

@@ -195,6 +195,46 @@ gfsm_status gfsm_query_roots(const gfsm_graph *data, const gfsm_graph *query, ui
     GUARD_END
 }
 
+gfsm_status gfsm_query_window(const gfsm_graph *data, const gfsm_graph *query, uint64_t max_first,
+                              uint32_t *root_lo, uint32_t *root_hi, uint64_t *first)
+{
+    GUARD_BEGIN
+    if (!data || !query || !root_lo || !root_hi) return fail(Status::err(GFSM_ERR_ARG, "null argument"));
+    Plan p;
+    Status st = plan_query(*query->g, p);
+    if (!st.good()) return fail(st);
+    const HostGraph &g = *data->g;
+    uint64_t lo, hi, f = 0;
+    *root_lo = *root_hi = g.V;  // empty window
+    if (p.q > 1) {
+        root_range(g, p, 0, 0, lo, hi);
+        // Walk down from the highest root id while the first-depth
+        // candidate total stays within max_first (at least one root).
+        uint64_t k = hi;
+        while (k > lo) {
+            const uint64_t c = g.rel_cnt[k - 1];
+            if (k < hi && f + c > max_first) break;
+            f += c;
+            k--;
+        }
+        if (k < hi) *root_lo = g.rel_vtx[k];
+    } else {  // one vertex: every vertex with a segment is a root
+        uint32_t x = g.V;
+        while (x > 0) {
+            const uint32_t y = x - 1;
+            if (g.degree(y)) {
+                if (f > 0 && f >= max_first) break;
+                f++;
+            }
+            x = y;
+        }
+        *root_lo = x;
+    }
+    if (first) *first = f;
+    return ok();
+    GUARD_END
+}
+
 gfsm_status gfsm_match(gfsm_ctx *ctx, const gfsm_graph *data, const gfsm_graph *query, const gfsm_match_opts *opts,
                        gfsm_result **r)
 {

@@ -303,6 +303,8 @@ struct Runner {
     uint32_t grid_cap = 0;
     uint64_t max_cand = 0;      // work cap (0 = none)
     uint64_t cand_total = 0;    // candidates launched so far
+    uint64_t max_depth_cand = 0;           // deterministic per-depth cap (0 = none)
+    uint64_t depth_done[GFSM_MAXQ] = {};   // candidates tested so far per depth (roots: depth 0)
     uint32_t forced_items = 0;  // from gfsm_match_opts.tile (0 = tuned)
 
     Runner(Ctx &c, const HostGraph &d, const DevGraph &g, const Plan &p, uint32_t f, uint64_t mr, Result &r)
@@ -413,8 +415,12 @@ struct Runner {
     {
         res.stats.roots = rhi - rlo;
         uint64_t step = std::min<uint64_t>(cap[0], launch_limit());
-        for (uint64_t c = rlo; c < rhi; c += step) {
+        for (uint64_t c = rlo; c < rhi;) {
             uint64_t e = std::min(rhi, c + step);
+            if (max_depth_cand) {
+                if (depth_done[0] >= max_depth_cand) return Status::err(GFSM_ERR_CAP, "per-depth candidate cap reached");
+                e = std::min(e, c + (max_depth_cand - depth_done[0]));
+            }
             RootArgs a{};
             a.g = dg.view;
             a.io.cand_vtx = dg.rel_vtx + c;
@@ -442,10 +448,13 @@ struct Runner {
             if (!st.good()) return st;
             const uint64_t S0 = ctx.lane.tot_h->keep, G1 = ctx.lane.tot_h->cnt;
             res.stats.level[0] += S0;
+            res.stats.cand[0] += e - c;  // root candidates tested
+            depth_done[0] += e - c;
             if (S0) {
                 st = descend(1, 0, G1, S0);
                 if (!st.good()) return st;
             }
+            c = e;
         }
         return Status::ok();
     }
@@ -461,7 +470,14 @@ struct Runner {
         const uint64_t tc = (uint64_t)kTile * items;
         bool first = true;
         for (uint64_t c = g0; c < g1;) {
-            const uint64_t e = std::min(g1, c + std::max<uint64_t>(tc, std::min(chunk[d], tile_limit)));
+            uint64_t e = std::min(g1, c + std::max<uint64_t>(tc, std::min(chunk[d], tile_limit)));
+            if (max_depth_cand) {
+                // Deterministic cap: depth d tests only its first max_depth_cand
+                // candidates in enumeration order; when it has none left, the
+                // match stops (every deeper range is finished by then).
+                if (depth_done[d] >= max_depth_cand) return Status::err(GFSM_ERR_CAP, "per-depth candidate cap reached");
+                e = std::min(e, c + (max_depth_cand - depth_done[d]));
+            }
             ExpandArgs a{};
             a.items = items;
             a.pool_words = pool_words;
@@ -511,6 +527,7 @@ struct Runner {
             if (!st.good()) return st;
             if (kind == EXPAND_LAST_COUNT) {
                 res.stats.cand[d] += e - c;
+                depth_done[d] += e - c;
                 c = e;
                 continue;
             }
@@ -526,6 +543,7 @@ struct Runner {
             }
             res.stats.cand[d] += e - c;
             res.stats.level[d] += keep;
+            depth_done[d] += e - c;
             if (keep) {
                 const double est = (double)limit * (double)(e - c) / (double)keep * 0.7;
                 chunk[d] = (uint64_t)std::max<double>((double)tc, std::min<double>(est, (double)tile_limit));
@@ -615,6 +633,7 @@ Status match_plain(Ctx &ctx, const HostGraph &data, const HostGraph &query, cons
 
     Runner run(ctx, data, *dg, plan, o.flags, o.max_rows, res);
     run.max_cand = o.max_cand;
+    run.max_depth_cand = o.max_depth_cand;
     if (o.tile) {
         if (o.tile != 256 && o.tile != 512 && o.tile != 1024 && o.tile != 2048)
             return Status::err(GFSM_ERR_ARG, "tile must be 0, 256, 512, 1024 or 2048 candidates");

@@ -58,7 +58,7 @@ struct Tuning {
     uint32_t batch_pool_words = kPoolWords;
     uint32_t batch_wgs_per_cu = 8;
     uint32_t batch_slots = 256;
-    uint32_t batch_steps = 8;  // steps per captured replay
+    uint32_t batch_steps = 4;  // steps per captured replay (the server uses at most 2)
     Tuning();
 };
 

@@ -161,6 +161,7 @@ struct BatchSlot {
     uint32_t *gbeg[GFSM_MAXQ];
     uint64_t cap[GFSM_MAXQ];
     uint64_t max_cand;            // work cap (0 = none)
+    uint64_t max_depth_cand;      // deterministic per-depth cap (0 = none)
     uint32_t q, digest;
     uint32_t state;               // SLOT_NEW at admission
     uint32_t error;
@@ -188,12 +189,23 @@ struct BatchCtl {
     uint64_t step_cand;           // candidates of one slot in one step (set by the host
                                   // between replays; 0 = max_step_cand)
     uint32_t running;             // slots in SLOT_NEW or SLOT_RUN after scheduling
-    uint32_t pad;
+    uint32_t n_admit;             // admission entries waiting (set by the host, cleared by the scheduler)
+};
+
+// One admission, written by the host into a contiguous queue between
+// replays; the scheduler installs it into its slot.
+struct AdmitEntry {
+    uint32_t slot;
+    uint32_t pad[3];
+    BatchSlot s;
+    QueryPlanDev plan;
 };
 
 // Batch configuration shared by the scheduler and the step kernel.
 struct BatchCfg {
     BatchSlot *slots;
+    QueryPlanDev *plans;          // [nslots], slot s's plan (BatchSlot::plan points here)
+    const AdmitEntry *admit;      // [nslots] admission queue
     SlotReport *report;           // [nslots]
     uint64_t *tile_start;         // [nslots + 1]
     BatchCtl *ctl;

@@ -760,6 +760,7 @@ DEV uint32_t schedule_slot(const BatchCfg &cfg, BatchSlot &B)
         const uint32_t d = B.d;
         if (B.step == STEP_ROOT) {
             B.root_next = e;
+            B.cand[0] += e - b;
             B.level[0] += t.keep;
             if (t.keep) {
                 B.S[0] = t.keep;
@@ -805,10 +806,20 @@ DEV uint32_t schedule_slot(const BatchCfg &cfg, BatchSlot &B)
     while (d >= 1 && B.lo[d] >= B.hi[d]) d--;
     uint32_t ntiles = 0;
     B.tot = Totals{};
+    // Deterministic cap: depth k tests only its first max_depth_cand
+    // candidates (B.cand[k], roots included); the next step's depth has none
+    // left -> the match stops (every deeper range is finished by then).
+    const uint32_t dk = (d >= 1) ? (uint32_t)d : 0u;
+    if ((d >= 1 || B.root_next < B.nroots) && B.max_depth_cand && B.cand[dk] >= B.max_depth_cand) {
+        B.state = SLOT_CAP;
+        B.step = STEP_NONE;
+        return 0;
+    }
     if (d >= 1) {
         const uint32_t kind = (d == (int32_t)q - 1) ? (uint32_t)EXPAND_LAST_COUNT : (uint32_t)EXPAND_MID;
         const uint64_t b = B.lo[d];
-        const uint64_t e = umin64(B.hi[d], b + umax64(TC, umin64(B.chunk[d], limit)));
+        uint64_t e = umin64(B.hi[d], b + umax64(TC, umin64(B.chunk[d], limit)));
+        if (B.max_depth_cand) e = umin64(e, b + (B.max_depth_cand - B.cand[d]));
         const DepthPlanDev &dp = B.plan->depth[d];
         if (B.S[d - 1] == 0 || B.S[d - 1] > B.cap[d - 1] || dp.gen_dist == 0 || dp.gen_dist > (uint32_t)d) {
             B.state = SLOT_ERROR;
@@ -850,7 +861,8 @@ DEV uint32_t schedule_slot(const BatchCfg &cfg, BatchSlot &B)
         ntiles = (uint32_t)io.ntiles;
     } else if (B.root_next < B.nroots) {
         const uint64_t b = B.root_next;
-        const uint64_t e = umin64(B.nroots, b + B.chunk[0]);
+        uint64_t e = umin64(B.nroots, b + B.chunk[0]);
+        if (B.max_depth_cand) e = umin64(e, b + (B.max_depth_cand - B.cand[0]));
         RootIO r{};
         r.cand_vtx = B.root_vtx + b;
         r.cand_beg = B.root_beg + b;
@@ -876,11 +888,36 @@ DEV uint32_t schedule_slot(const BatchCfg &cfg, BatchSlot &B)
     return ntiles;
 }
 
-// One workgroup: schedule every slot, then lay the slots' tiles end to end.
+// Copy `words` 32-bit words with all threads of the workgroup.
+DEV void block_copy(uint32_t *__restrict__ dst, const uint32_t *__restrict__ src, uint32_t words)
+{
+    for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) dst[i] = src[i];
+}
+
+// One workgroup: install queued admissions, schedule every slot, then lay
+// the slots' tiles end to end.
 __global__ __launch_bounds__(kTile) void k_batch_schedule(BatchCfg cfg)
 {
     __shared__ uint32_t s_wt[kWaves], s_wr[kWaves];
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+    const uint32_t n_admit = cfg.ctl->n_admit;
+    if (n_admit) {
+        static_assert(sizeof(BatchSlot) % 4 == 0 && sizeof(QueryPlanDev) % 4 == 0, "word copies");
+        for (uint32_t e = 0; e < n_admit && e < cfg.nslots; e++) {
+            const AdmitEntry &a = cfg.admit[e];
+            const uint32_t s = a.slot;
+            if (s >= cfg.nslots) continue;
+            block_copy(reinterpret_cast<uint32_t *>(cfg.slots + s), reinterpret_cast<const uint32_t *>(&a.s),
+                       sizeof(BatchSlot) / 4);
+            block_copy(reinterpret_cast<uint32_t *>(cfg.plans + s), reinterpret_cast<const uint32_t *>(&a.plan),
+                       sizeof(QueryPlanDev) / 4);
+        }
+        // The copies are read back below by other threads of this workgroup.
+        __threadfence();
+        __syncthreads();
+        __threadfence();
+        if (tid == 0) cfg.ctl->n_admit = 0;
+    }
     const uint32_t per = (cfg.nslots + kTile - 1) / kTile;  // <= kMaxSlots / kTile
     const uint32_t s0 = tid * per;
     uint32_t nt[kMaxSlots / kTile];

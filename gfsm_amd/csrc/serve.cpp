@@ -12,6 +12,7 @@
 // has its own levels, its own scan range and its own totals.
 #include "serve.h"
 
+#include <algorithm>
 #include <chrono>
 
 namespace gfsm {
@@ -138,7 +139,8 @@ Status server_start(Ctx &ctx, const HostGraph &data, const gfsm_serve_opts &opts
     sh.items = ctx.tune.batch_items;
     sh.pool_words = ctx.tune.batch_pool_words;
     sh.wgs_per_cu = ctx.tune.batch_wgs_per_cu;
-    sh.steps_per_replay = ctx.tune.batch_steps;
+    // Short replays: a finished slot is refilled at the end of its replay.
+    sh.steps_per_replay = std::min<uint32_t>(ctx.tune.batch_steps, 2);
     if (sh.slots > kMaxSlots) return Status::err(GFSM_ERR_ARG, "max_inflight must be at most 1024");
     const uint64_t budget = opts.mem_budget ? opts.mem_budget : ctx.budget;
     s->eng.reset(new BatchEngine(ctx, data));

@@ -109,6 +109,9 @@ private:
     uint64_t steps_ = 0, since_clear_ = 0;
     uint64_t step_cand_ = 0;             // per-slot step size last written to the device
     uint64_t *step_cand_h_ = nullptr;    // pinned source of that write
+    AdmitEntry *admit_h_ = nullptr, *admit_d_ = nullptr;  // admission queue (pinned, device)
+    uint32_t *n_admit_h_ = nullptr;
+    uint32_t n_admit_ = 0;               // entries queued since the last replay
     double device_ms_ = 0;
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t exec_ = nullptr;

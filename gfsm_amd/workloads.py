@@ -116,8 +116,11 @@ def config4_queries(data, n=256):
 # CONFIG4_ROOT_K[i] root candidates by id, the largest of n, n/4, n/16, ... (n =
 # the query's root candidates) whose count finishes within 5e7 candidates.
 # The queries in CONFIG4_CAPPED have no root whose subtree fits that bound;
-# they run over their last root under the work cap CONFIG4_CAP and report a
-# partial (capped) count.
+# they run over their last root with the deterministic cap CONFIG4_DEPTH_CAP
+# (every depth tests only its first that many candidates in enumeration order)
+# and report a partial (capped) count that is the same on every path.
+# CONFIG4_CAP is a hard bound on all candidates of a query, above
+# q x CONFIG4_DEPTH_CAP, so it is never what stops a query.
 CONFIG4_ROOT_K = (
     1, 208, 358, 5, 16, 163, 1, 62, 1, 440, 1, 3, 1, 1, 1, 6402, 1, 23, 3554, 1, 633, 6804, 10, 7, 62, 3, 1, 83,
     16, 322, 268, 7, 1134, 12870, 6277, 2, 441, 6, 9, 4, 6552, 4, 12246, 7, 8575, 1, 6804, 16, 24, 16, 437, 7,
@@ -133,7 +136,8 @@ CONFIG4_ROOT_K = (
 CONFIG4_CAPPED = (
     0, 8, 10, 12, 14, 26, 45, 58, 61, 65, 69, 91, 93, 103, 125, 128, 129, 131, 150, 166, 171, 174, 175, 188, 195,
     213, 222, 229, 248, 255)
-CONFIG4_CAP = 50_000_000
+CONFIG4_DEPTH_CAP = 1 << 22
+CONFIG4_CAP = 1 << 30
 
 
 def config4_windows(data, queries, ks=CONFIG4_ROOT_K):
@@ -147,9 +151,9 @@ def config4_windows(data, queries, ks=CONFIG4_ROOT_K):
 
 
 def config4_caps(n=256):
-    """Per-query work caps (0 = none) of the batched workload."""
+    """Per-query (max_cand, max_depth_cand) of the batched workload."""
     capped = set(CONFIG4_CAPPED)
-    return [CONFIG4_CAP if i in capped else 0 for i in range(n)]
+    return [(CONFIG4_CAP, CONFIG4_DEPTH_CAP if i in capped else 0) for i in range(n)]
 
 
 def config5_data(scale=23):
@@ -167,6 +171,22 @@ def config5_queries(data, n=10000):
     sizes = config5_sizes(n)
     return [Graph.walk_query(data, int(sizes[i]), kind=WALK_SPARSE if i % 2 else WALK_DENSE, seed=5001 + i)
             for i in range(n)]
+
+
+# Bounded work per served query: the highest-id roots whose depth-1
+# candidates total at most CONFIG5_FIRST (at least one root), and the
+# deterministic cap CONFIG5_DEPTH_CAP (every depth tests only its first that
+# many candidates in enumeration order); a query that reaches it reports a
+# partial (capped) count that is the same served and serial.  CONFIG5_CAP
+# bounds all candidates of a query above q x CONFIG5_DEPTH_CAP (never reached).
+CONFIG5_FIRST = 256
+CONFIG5_DEPTH_CAP = 1 << 20
+CONFIG5_CAP = 1 << 28
+
+
+def config5_windows(data, queries, max_first=CONFIG5_FIRST):
+    """Root-id windows [lo, hi) of the served stream's queries."""
+    return [data.window(q, max_first)[:2] for q in queries]
 
 
 def poisson_arrivals(n, rate, seed=5999):

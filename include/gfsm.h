@@ -85,7 +85,13 @@ typedef struct gfsm_match_opts {
     uint64_t max_rows;          /* cap on materialised rows (0 = no cap)                          */
     uint64_t mem_budget;        /* device bytes for intermediate levels (0 = context default)     */
     uint64_t max_cand;          /* stop after this many candidates (0 = no cap): the call returns */
-                                /* GFSM_ERR_CAP and a partial result whose status says so         */
+                                /* GFSM_ERR_CAP and a partial result whose status says so; where  */
+                                /* it stops depends on how the work was chunked                   */
+    uint64_t max_depth_cand;    /* deterministic cap (0 = none): every depth (roots included)      */
+                                /* tests only its first max_depth_cand candidates in enumeration   */
+                                /* order (root id, then segment order at every depth); the match   */
+                                /* stops at the first depth that runs out.  The partial count and  */
+                                /* digest are the same on every path and chunking; GFSM_ERR_CAP    */
 } gfsm_match_opts;
 
 typedef struct gfsm_stats {
@@ -95,7 +101,7 @@ typedef struct gfsm_stats {
     uint32_t overflows;         /* launches discarded because a level overflowed its buffer  */
     uint64_t roots;             /* root candidates read                                      */
     uint64_t level[GFSM_MAX_Q]; /* nodes kept per depth (last depth: embeddings)             */
-    uint64_t cand[GFSM_MAX_Q];  /* candidates generated per depth                            */
+    uint64_t cand[GFSM_MAX_Q];  /* candidates tested per depth (depth 0: root candidates)     */
     uint32_t checks[GFSM_MAX_Q];/* membership probes per candidate at each depth             */
     uint64_t alg_bytes;         /* algorithmic HBM bytes of the path (see design notes)      */
     double device_ms;           /* first matcher kernel -> result ready (HIP events)         */
@@ -146,6 +152,14 @@ gfsm_status gfsm_plan_query(const gfsm_graph *query, uint32_t *q, uint32_t *maxv
  * start an embedding.  Pass ids = NULL to learn n first.  (No counterpart: the
  * reference starts from every row, HnGPU.cu:475-495.) */
 gfsm_status gfsm_query_roots(const gfsm_graph *data, const gfsm_graph *query, uint32_t *ids, uint64_t *n);
+
+/* A root window for bounded work (no counterpart): the largest set of the
+ * highest-id root candidates whose depth-1 candidates total at most max_first
+ * (always at least one root), as [*root_lo, *root_hi) for gfsm_match_opts;
+ * *first (may be NULL) = that total.  Deterministic: depends only on the
+ * graphs. */
+gfsm_status gfsm_query_window(const gfsm_graph *data, const gfsm_graph *query, uint64_t max_first,
+                              uint32_t *root_lo, uint32_t *root_hi, uint64_t *first);
 
 /* ---- matching ------------------------------------------------------------ */
 gfsm_status gfsm_match(gfsm_ctx *ctx, const gfsm_graph *data, const gfsm_graph *query,

@@ -165,3 +165,36 @@ def test_autotune_keeps_results(ctx, er_vlabel):
         assert [(ctx.match(data, q, digest=True).count) for q in qs] == [r.count for r in ref]
     finally:
         ctx.set_tuning(**{k: v for k, v in saved.items()})
+
+
+def test_deterministic_last_cap_same_on_every_path(ctx):
+    """max_depth_cand cuts the enumeration at the same candidate of every depth
+    on every path: single query with large or tiny level buffers (different
+    chunking), batched with different slot counts, and served."""
+    data = G.Graph.rmat(12, 8, nvlabels=3, zipf_s=1.0, nelabels=2, seed=91, mode=G.MODE_VLABEL)
+    qs = _queries(data, [5, 6, 7, 6, 5, 7], seed=9100)
+    full = [ctx.match(data, q, digest=True) for q in qs]
+    for M in (1, 777, 50_000, 3_000_000):
+        a = [ctx.match(data, q, digest=True, max_depth_cand=M, allow_partial=True) for q in qs]
+        b = [ctx.match(data, q, digest=True, max_depth_cand=M, allow_partial=True, mem_budget=16 << 20) for q in qs]
+        assert [(x.count, x.digest, x.status) for x in a] == [(x.count, x.digest, x.status) for x in b], M
+        for x, f in zip(a, full):
+            most = max([f.stats["roots"]] + f.stats["cand"][1:f.stats["q"]])
+            if most <= M:
+                assert (x.status, x.count, x.digest) == (0, f.count, f.digest)
+            else:
+                assert x.status == 5 and x.count <= f.count
+        saved = ctx.tuning()
+        try:
+            for slots in (2, 6):
+                ctx.set_tuning(batch_slots=slots)
+                c = ctx.match_batch(data, qs, digest=True, max_depth_cand=M)
+                assert [(x.count, x.digest, x.status) for x in c] == [(x.count, x.digest, x.status) for x in a], (M, slots)
+        finally:
+            ctx.set_tuning(batch_slots=saved["batch_slots"])
+        srv = G.Server(ctx, data, max_inflight=3, digest=True)
+        try:
+            got = [srv.wait(t) for t in [srv.submit(q, max_depth_cand=M) for q in qs]]
+        finally:
+            srv.stop()
+        assert [(x.count, x.digest, x.status) for x in got] == [(x.count, x.digest, x.status) for x in a], M

@@ -72,13 +72,15 @@ class Stats(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("items", (C.c_uint32 * 32) * 3), ("pool_words", C.c_uint32), ("wgs_per_cu", C.c_uint32),
                 ("batch_items", C.c_uint32), ("batch_pool_words", C.c_uint32), ("batch_wgs_per_cu", C.c_uint32),
-                ("batch_slots", C.c_uint32)]
+                ("batch_slots", C.c_uint32), ("batch_steps", C.c_uint32), ("pad", C.c_uint32),
+                ("batch_step_budget", C.c_uint64)]
 
     def as_dict(self):
         return {"items": [list(self.items[k]) for k in range(3)], "pool_words": self.pool_words,
                 "wgs_per_cu": self.wgs_per_cu, "batch_items": self.batch_items,
                 "batch_pool_words": self.batch_pool_words, "batch_wgs_per_cu": self.batch_wgs_per_cu,
-                "batch_slots": self.batch_slots}
+                "batch_slots": self.batch_slots, "batch_steps": self.batch_steps,
+                "batch_step_budget": self.batch_step_budget}
 
 
 class ServeOpts(C.Structure):

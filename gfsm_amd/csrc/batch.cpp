@@ -438,9 +438,12 @@ Status BatchEngine::replay()
     if (trace_) {
         uint64_t steps_sum = 0;
         for (uint32_t s = 0; s < shape_.slots; s++) steps_sum += report_h_[s].steps;
-        std::fprintf(stderr, "batch replay %llu: running %u step_cand %llu device %.3f ms slot-steps %llu\n",
+        BatchCtl c{};
+        (void)hipMemcpy(&c, ctl_d_, sizeof(BatchCtl), hipMemcpyDeviceToHost);
+        std::fprintf(stderr,
+                     "batch replay %llu: running %u step_cand %llu device %.3f ms slot-steps %llu next-step tiles %llu\n",
                      (unsigned long long)(steps_ / K), running_, (unsigned long long)step_cand_, ms,
-                     (unsigned long long)steps_sum);
+                     (unsigned long long)steps_sum, (unsigned long long)c.total_tiles);
     }
     return Status::ok();
 }
@@ -521,6 +524,7 @@ static BatchShape shape_from(const Ctx &ctx)
     sh.pool_words = ctx.tune.batch_pool_words;
     sh.wgs_per_cu = ctx.tune.batch_wgs_per_cu;
     sh.steps_per_replay = ctx.tune.batch_steps;
+    sh.step_budget = ctx.tune.batch_step_budget;
     return sh;
 }
 

@@ -351,6 +351,9 @@ gfsm_status gfsm_tuning_get(gfsm_ctx *ctx, gfsm_tuning *t)
     t->batch_pool_words = u.batch_pool_words;
     t->batch_wgs_per_cu = u.batch_wgs_per_cu;
     t->batch_slots = u.batch_slots;
+    t->batch_steps = u.batch_steps;
+    t->pad = 0;
+    t->batch_step_budget = u.batch_step_budget;
     return ok();
     GUARD_END
 }
@@ -371,6 +374,10 @@ gfsm_status gfsm_tuning_set(gfsm_ctx *ctx, const gfsm_tuning *t)
         return fail(Status::err(GFSM_ERR_ARG, "workgroups per CU must be 1..32"));
     if (t->batch_slots < 1 || t->batch_slots > kMaxSlots)
         return fail(Status::err(GFSM_ERR_ARG, "batch slots must be 1..1024"));
+    if (t->batch_steps < 1 || t->batch_steps > 64)
+        return fail(Status::err(GFSM_ERR_ARG, "batch steps per replay must be 1..64"));
+    if (t->batch_step_budget < (1ull << 16))
+        return fail(Status::err(GFSM_ERR_ARG, "batch step budget must be at least 65536 candidates"));
     std::lock_guard<std::mutex> lk(ctx->c->mu);
     Tuning &u = ctx->c->tune;
     for (int k = 0; k < 3; k++)
@@ -381,6 +388,8 @@ gfsm_status gfsm_tuning_set(gfsm_ctx *ctx, const gfsm_tuning *t)
     u.batch_pool_words = t->batch_pool_words;
     u.batch_wgs_per_cu = t->batch_wgs_per_cu;
     u.batch_slots = t->batch_slots;
+    u.batch_steps = t->batch_steps;
+    u.batch_step_budget = t->batch_step_budget;
     return ok();
     GUARD_END
 }

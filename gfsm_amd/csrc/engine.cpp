@@ -121,6 +121,7 @@ static void free_mirror(DevGraph *dg)
     (void)hipFree(dg->rel_vtx);
     (void)hipFree(dg->rel_cnt);
     (void)hipFree(dg->rel_beg);
+    (void)hipFree(dg->seg);
     delete dg;
 }
 
@@ -213,7 +214,22 @@ Status upload_graph(Ctx &ctx, const HostGraph &g, DevGraph *&out)
         free_mirror(dg);
         return st;
     }
-    dg->view = DevGraphView{dg->nbr, dg->dir_ptr, dg->dir, g.V, (uint32_t)g.nent()};
+    // Segment hash table: a power of two at least twice the directory size.
+    uint64_t slots = 1024;
+    while (slots < 2 * (uint64_t)g.ndir()) slots <<= 1;
+    if (slots > (1ull << 32)) {
+        free_mirror(dg);
+        return Status::err(GFSM_ERR_ARG, "graph has too many (vertex, relation) segments");
+    }
+    hipError_t he = hipMalloc(&dg->seg, slots * sizeof(uint4));
+    if (he == hipSuccess) he = launch_seg_hash_build(dg->dir_ptr, dg->dir, g.V, dg->seg, (uint32_t)(slots - 1), 0);
+    if (he == hipSuccess) he = hipDeviceSynchronize();
+    if (he != hipSuccess) {
+        free_mirror(dg);
+        return hip_status(he, "segment hash table");
+    }
+    dg->bytes += slots * sizeof(uint4);
+    dg->view = DevGraphView{dg->nbr, dg->dir_ptr, dg->dir, dg->seg, (uint32_t)(slots - 1), g.V, (uint32_t)g.nent(), 0};
     g.dev[&ctx] = dg;
     ctx.graphs.insert(&g);
     out = dg;
@@ -280,6 +296,11 @@ Status match_single_vertex(const HostGraph &data, const Plan &plan, const gfsm_m
 }
 
 namespace {
+
+// First chunk of a capped match = cap / kCapStartDiv candidates: each launch
+// costs a host round trip, so the single-query path starts larger than the
+// batch engine (whose steps cost nothing per query).
+constexpr uint64_t kCapStartDiv = 8;
 
 // One plain match in flight: level buffers carved from the arena, the lane,
 // and the depth-first chunk recursion.  Level k (nodes with k+1 placed
@@ -384,6 +405,20 @@ struct Runner {
         return Status::ok();
     }
 
+    // Under a per-depth cap: the next chunk of depth d sized so its children
+    // roughly fill depth d+1's remaining budget, from this chunk's fan-out
+    // (cnt children of n candidates).  Only the amount of wasted work
+    // changes; the capped result does not depend on chunk sizes.
+    uint64_t capped_chunk(uint32_t d, uint64_t n, uint64_t cnt, uint64_t chunk_now, uint64_t tmin) const
+    {
+        if (d + 1 >= q) return chunk_now;
+        const uint64_t used = depth_done[d + 1];
+        const uint64_t rem = max_depth_cand > used ? max_depth_cand - used : 0;
+        const uint64_t after = rem > cnt ? rem - cnt : 0;
+        const uint64_t need = (uint64_t)((double)after * (double)n / (double)std::max<uint64_t>(cnt, 1)) + 1;
+        return std::max<uint64_t>(tmin, std::min<uint64_t>(chunk_now, need));
+    }
+
     Status sync_totals()
     {
         HIP_TRY(hipMemcpyAsync(ctx.lane.tot_h, ctx.lane.tot_d, sizeof(Totals), hipMemcpyDeviceToHost,
@@ -415,6 +450,7 @@ struct Runner {
     {
         res.stats.roots = rhi - rlo;
         uint64_t step = std::min<uint64_t>(cap[0], launch_limit());
+        if (max_depth_cand) step = std::max<uint64_t>(kRootTile, std::min<uint64_t>(step, max_depth_cand / kCapStartDiv));
         for (uint64_t c = rlo; c < rhi;) {
             uint64_t e = std::min(rhi, c + step);
             if (max_depth_cand) {
@@ -450,6 +486,7 @@ struct Runner {
             res.stats.level[0] += S0;
             res.stats.cand[0] += e - c;  // root candidates tested
             depth_done[0] += e - c;
+            if (max_depth_cand) step = capped_chunk(0, e - c, G1, step, kRootTile);
             if (S0) {
                 st = descend(1, 0, G1, S0);
                 if (!st.good()) return st;
@@ -547,6 +584,8 @@ struct Runner {
             if (keep) {
                 const double est = (double)limit * (double)(e - c) / (double)keep * 0.7;
                 chunk[d] = (uint64_t)std::max<double>((double)tc, std::min<double>(est, (double)tile_limit));
+                if (max_depth_cand && kind == EXPAND_MID)
+                    chunk[d] = capped_chunk(d, e - c, ctx.lane.tot_h->cnt, chunk[d], tc);
             } else {
                 chunk[d] = std::min<uint64_t>(chunk[d] * 4, tile_limit);
             }
@@ -641,6 +680,9 @@ Status match_plain(Ctx &ctx, const HostGraph &data, const HostGraph &query, cons
     }
     st = run.carve(o.mem_budget ? o.mem_budget : ctx.budget);
     if (!st.good()) return st;
+    if (o.max_depth_cand)  // capped: start with an eighth of the cap, then follow the observed fan-out
+        for (uint32_t k = 0; k < GFSM_MAXQ; k++)
+            run.chunk[k] = std::max<uint64_t>(2048, std::min<uint64_t>(run.chunk[k], o.max_depth_cand / kCapStartDiv));
     uint64_t rlo, rhi;
     root_range(data, plan, o.root_lo, o.root_hi, rlo, rhi);
 

@@ -18,6 +18,7 @@ struct DevGraph {
     const void *ctx = nullptr;
     uint32_t *nbr = nullptr, *dir_ptr = nullptr, *dir = nullptr, *rel_vtx = nullptr, *rel_beg = nullptr,
              *rel_cnt = nullptr;
+    uint4 *seg = nullptr;  // segment hash table (see DevGraphView)
     DevGraphView view{};
     size_t bytes = 0;
 };
@@ -54,11 +55,12 @@ struct Tuning {
     uint8_t items[3][GFSM_MAXQ];
     uint32_t pool_words = kPoolWords;
     uint32_t wgs_per_cu = 8;
-    uint32_t batch_items = 4;
-    uint32_t batch_pool_words = kPoolWords;
-    uint32_t batch_wgs_per_cu = 8;
+    uint32_t batch_items = 8;
+    uint32_t batch_pool_words = 14336;
+    uint32_t batch_wgs_per_cu = 4;
     uint32_t batch_slots = 256;
     uint32_t batch_steps = 4;  // steps per captured replay (the server uses at most 2)
+    uint64_t batch_step_budget = 1ull << 26;
     Tuning();
 };
 

@@ -11,12 +11,21 @@
 namespace gfsm {
 
 // Device view of a resident data graph (see HostGraph for the layout).
+// Segments are found through `seg`, an open-addressing hash table keyed by
+// (vertex, relation): slot = {vertex, relation, begin, end}, vertex ~0u =
+// empty, at most half full, linear probing from seg_hash(vertex, relation).
+// One probe usually suffices where a directory search takes log2(relations
+// of the vertex) dependent loads.
+constexpr uint32_t kSegEmpty = 0xFFFFFFFFu;
 struct DevGraphView {
     const uint32_t *nbr;      // [nent]   targets, grouped per (vertex, relation), ascending
     const uint32_t *dir_ptr;  // [V+1]    directory range of each vertex
     const uint32_t *dir;      // [2*(ndir+1)] {relation, begin} pairs + sentinel
+    const uint4 *seg;         // [seg_mask+1] segment hash table
+    uint32_t seg_mask;
     uint32_t V;
     uint32_t nent;            // entries in nbr (bounds guard)
+    uint32_t pad;
 };
 
 // Chained-scan workspace of one launch lane (one stream).  Each tile
@@ -231,6 +240,11 @@ hipError_t launch_batch_step(const BatchCfg &cfg, uint32_t grid, hipStream_t s);
 hipError_t launch_scan_test(const uint32_t *vals, uint64_t n, uint64_t *out, uint32_t *idx,
                             ScanWS ws, Totals *tot, uint32_t grid, hipStream_t s);
 constexpr uint32_t kScanTestItems = 2;
+
+// Build the segment hash table of a graph: fill `slots` (mask+1 entries)
+// with empty slots, then insert every directory entry.
+hipError_t launch_seg_hash_build(const uint32_t *dir_ptr, const uint32_t *dir, uint32_t V, uint4 *slots,
+                                 uint32_t mask, hipStream_t s);
 
 // Zero the control block of a workspace (device kernel; safe under capture).
 hipError_t launch_ws_reset(ScanWS ws, Totals *tot, hipStream_t s);

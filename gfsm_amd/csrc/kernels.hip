@@ -78,57 +78,54 @@ DEV uint64_t mix64(uint64_t z)
 // graph access, I lookups per thread in lockstep
 // ---------------------------------------------------------------------------
 
+// Hash of a (vertex, relation) key (the "lowbias32" integer mix).
+DEV uint32_t seg_hash(uint32_t x, uint32_t rel)
+{
+    uint32_t h = x * 0x9E3779B1u + rel * 0x85EBCA6Bu;
+    h ^= h >> 16;
+    h *= 0x7FEB352Du;
+    h ^= h >> 15;
+    h *= 0x846CA68Bu;
+    h ^= h >> 16;
+    return h;
+}
+
 // seg[k] = [beg, end) of the (x[k], rel) segment, {0, 0} when x[k] has none
-// or act[k] is false.  Finds the last directory entry of x[k] whose relation
-// is <= rel with a branch-free halving search; all I searches issue their
-// loads together.
+// or act[k] is false: linear probing in the segment hash table, the I
+// lookups issuing their probes together.
 template <uint32_t I>
 DEV void seg_lookup_n(const DevGraphView &g, const uint32_t (&x)[I], uint32_t rel, const bool (&act)[I],
                       uint2 (&seg)[I])
 {
-    const uint2 *dir = reinterpret_cast<const uint2 *>(g.dir);
-    uint32_t lo[I], n[I];
+    uint32_t p[I];
+    bool pend[I];
 #pragma unroll
     for (uint32_t k = 0; k < I; k++) {
-        lo[k] = 0;
-        n[k] = 0;
-        if (act[k]) {
-            lo[k] = g.dir_ptr[x[k]];
-            n[k] = g.dir_ptr[x[k] + 1];
-        }
+        pend[k] = act[k];
+        p[k] = seg_hash(x[k], rel) & g.seg_mask;
+        seg[k] = make_uint2(0u, 0u);
     }
-#pragma unroll
-    for (uint32_t k = 0; k < I; k++) n[k] -= lo[k];
     for (;;) {
         bool more = false;
 #pragma unroll
-        for (uint32_t k = 0; k < I; k++) more |= n[k] > 1;
+        for (uint32_t k = 0; k < I; k++) more |= pend[k];
         if (!more) break;
-        uint32_t key[I];
+        uint4 e[I];
 #pragma unroll
-        for (uint32_t k = 0; k < I; k++) key[k] = (n[k] > 1) ? dir[lo[k] + (n[k] >> 1)].x : 0u;
+        for (uint32_t k = 0; k < I; k++) e[k] = pend[k] ? g.seg[p[k]] : make_uint4(kSegEmpty, 0u, 0u, 0u);
 #pragma unroll
         for (uint32_t k = 0; k < I; k++) {
-            if (n[k] > 1) {
-                const uint32_t h = n[k] >> 1;
-                if (key[k] <= rel) lo[k] += h;
-                n[k] -= h;
+            if (!pend[k]) continue;
+            if (e[k].x == x[k] && e[k].y == rel) {
+                seg[k] = make_uint2(e[k].z, e[k].w);
+                pend[k] = false;
+            } else if (e[k].x == kSegEmpty) {
+                pend[k] = false;  // absent
+            } else {
+                p[k] = (p[k] + 1) & g.seg_mask;
             }
         }
     }
-    uint2 e[I];
-    uint32_t nb[I];
-#pragma unroll
-    for (uint32_t k = 0; k < I; k++) {
-        e[k] = make_uint2(~0u, 0u);
-        nb[k] = 0;
-        if (n[k]) {
-            e[k] = dir[lo[k]];
-            nb[k] = dir[lo[k] + 1].y;  // the directory ends with a sentinel entry
-        }
-    }
-#pragma unroll
-    for (uint32_t k = 0; k < I; k++) seg[k] = (e[k].x == rel) ? make_uint2(e[k].y, nb[k]) : make_uint2(0u, 0u);
 }
 
 // keep[k] &= (w[k] is in the sorted slice nbr[seg[k].x, seg[k].y)): the last
@@ -727,6 +724,22 @@ __global__ __launch_bounds__(kTile) void k_expand(ExpandArgs a)
 DEV uint64_t umin64(uint64_t a, uint64_t b) { return a < b ? a : b; }
 DEV uint64_t umax64(uint64_t a, uint64_t b) { return a > b ? a : b; }
 
+// Under a per-depth cap, the next chunk of depth d should generate about as
+// many depth-(d+1) candidates as that depth has budget left once this
+// chunk's `cnt` children are tested: estimated from this chunk's fan-out
+// (cnt children from `n` candidates).  Fewer candidates wasted beyond the
+// cap; the result does not depend on chunk sizes.
+DEV uint64_t capped_chunk(const BatchSlot &B, uint32_t d, uint64_t n, uint64_t cnt, uint64_t chunk, uint64_t TC,
+                          uint64_t limit)
+{
+    if (d + 1 >= B.q) return chunk;
+    const uint64_t used = B.cand[d + 1];
+    const uint64_t rem = B.max_depth_cand > used ? B.max_depth_cand - used : 0;
+    const uint64_t after = rem > cnt ? rem - cnt : 0;
+    const uint64_t need = after / umax64(cnt, 1) * n + after % umax64(cnt, 1) * n / umax64(cnt, 1) + 1;
+    return umax64(TC, umin64(umin64(chunk, need), limit));
+}
+
 // Consume the finished step of slot B (depth-first chunk walk), then
 // schedule its next step.  Returns the new step's tile count (0 when the slot
 // has no step).  Mirrors the single-query driver's recursion: a level that
@@ -748,6 +761,8 @@ DEV uint32_t schedule_slot(const BatchCfg &cfg, BatchSlot &B)
             B.chunk[k] = (k + 1 < q) ? umin64(B.cap[k], limit) : limit;
         }
         B.chunk[0] = umax64(TC, umin64(B.cap[0], limit));
+        if (B.max_depth_cand)  // capped: start small, grow from the observed fan-out
+            for (uint32_t k = 0; k < GFSM_MAXQ; k++) B.chunk[k] = umax64(TC, umin64(B.chunk[k], B.max_depth_cand / 64));
         B.cand_done = 0;
         B.count = B.dg0 = B.dg1 = 0;
         B.steps = B.overflows = 0;
@@ -767,6 +782,7 @@ DEV uint32_t schedule_slot(const BatchCfg &cfg, BatchSlot &B)
                 B.lo[1] = 0;
                 B.hi[1] = t.cnt;
             }
+            if (B.max_depth_cand) B.chunk[0] = capped_chunk(B, 0, e - b, t.cnt, B.chunk[0], TC, limit);
             B.steps++;
         } else if (B.step == EXPAND_MID) {
             B.steps++;
@@ -780,6 +796,7 @@ DEV uint32_t schedule_slot(const BatchCfg &cfg, BatchSlot &B)
                 if (t.keep) {
                     const uint64_t est = B.cap[d] / 10 * 7 * (e - b) / t.keep;
                     B.chunk[d] = umax64(TC, umin64(est, limit));
+                    if (B.max_depth_cand) B.chunk[d] = capped_chunk(B, d, e - b, t.cnt, B.chunk[d], TC, limit);
                     B.S[d] = t.keep;
                     B.lo[d + 1] = 0;
                     B.hi[d + 1] = t.cnt;
@@ -1063,6 +1080,39 @@ __global__ __launch_bounds__(kTile) void k_scan_test(const uint32_t *vals, uint6
     finish_launch(ws);
 }
 
+__global__ void k_seg_fill(uint4 *slots, uint32_t mask)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= mask;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        slots[i] = make_uint4(kSegEmpty, 0u, 0u, 0u);
+}
+
+// One thread per vertex inserts its directory entries; a slot is claimed by
+// a compare-and-swap on its vertex word (keys are unique, and the table is
+// at most half full, so every insertion finds a free slot).
+__global__ void k_seg_insert(const uint32_t *dir_ptr, const uint32_t *dir, uint32_t V, uint4 *slots, uint32_t mask)
+{
+    for (uint64_t xi = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; xi < V;
+         xi += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t x = (uint32_t)xi;
+        const uint32_t jb = dir_ptr[x], je = dir_ptr[x + 1];
+        for (uint32_t j = jb; j < je; j++) {
+            const uint32_t rel = dir[2 * j], beg = dir[2 * j + 1], end = dir[2 * j + 3];
+            uint32_t p = seg_hash(x, rel) & mask;
+            for (;;) {
+                uint32_t *key = reinterpret_cast<uint32_t *>(slots + p);
+                if (atomicCAS(key, kSegEmpty, x) == kSegEmpty) {
+                    key[1] = rel;
+                    key[2] = beg;
+                    key[3] = end;
+                    break;
+                }
+                p = (p + 1) & mask;
+            }
+        }
+    }
+}
+
 __global__ void k_ws_reset(ScanWS ws, Totals *tot)
 {
     if (threadIdx.x < 4) ws.ctl[threadIdx.x] = 0u;
@@ -1175,6 +1225,16 @@ hipError_t launch_scan_test(const uint32_t *vals, uint64_t n, uint64_t *out, uin
     const uint64_t tc = (uint64_t)kTile * kScanTestItems;
     const uint64_t ntiles = (n + tc - 1) / tc;
     hipLaunchKernelGGL(k_scan_test, dim3(grid), dim3(kTile), 0, s, vals, n, out, idx, ws, tot, ntiles);
+    return hipGetLastError();
+}
+
+hipError_t launch_seg_hash_build(const uint32_t *dir_ptr, const uint32_t *dir, uint32_t V, uint4 *slots,
+                                 uint32_t mask, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_seg_fill, dim3(2048), dim3(256), 0, s, slots, mask);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_seg_insert, dim3(4096), dim3(256), 0, s, dir_ptr, dir, V, slots, mask);
     return hipGetLastError();
 }
 

@@ -141,6 +141,7 @@ Status server_start(Ctx &ctx, const HostGraph &data, const gfsm_serve_opts &opts
     sh.wgs_per_cu = ctx.tune.batch_wgs_per_cu;
     // Short replays: a finished slot is refilled at the end of its replay.
     sh.steps_per_replay = std::min<uint32_t>(ctx.tune.batch_steps, 2);
+    sh.step_budget = ctx.tune.batch_step_budget;
     if (sh.slots > kMaxSlots) return Status::err(GFSM_ERR_ARG, "max_inflight must be at most 1024");
     const uint64_t budget = opts.mem_budget ? opts.mem_budget : ctx.budget;
     s->eng.reset(new BatchEngine(ctx, data));

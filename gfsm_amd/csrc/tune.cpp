@@ -102,6 +102,7 @@ Status autotune(Ctx &ctx, const HostGraph &data, const std::vector<const HostGra
         BatchShape base;
         base.slots = ctx.tune.batch_slots;
         base.steps_per_replay = ctx.tune.batch_steps;
+        base.step_budget = ctx.tune.batch_step_budget;
         double best = 1e300;
         BatchShape pick = base;
         pick.items = ctx.tune.batch_items;

@@ -223,7 +223,9 @@ gfsm_status gfsm_selftest_scan(gfsm_ctx *ctx, const uint32_t *vals, uint64_t n, 
  * depth writing rows, at depth d.  pool_words: LDS words per workgroup that
  * stage the tile's frontier nodes (checked segments, placed vertices).
  * wgs_per_cu: persistent workgroups per compute unit.  batch_*: the same for
- * the batch step kernel; batch_slots: queries in flight in one batch. */
+ * the batch step kernel; batch_slots: queries in flight in one batch;
+ * batch_steps: steps per captured replay; batch_step_budget: candidates one
+ * step processes over all running queries (split evenly among them). */
 typedef struct gfsm_tuning {
     uint32_t items[3][GFSM_MAX_Q];
     uint32_t pool_words;
@@ -232,6 +234,9 @@ typedef struct gfsm_tuning {
     uint32_t batch_pool_words;
     uint32_t batch_wgs_per_cu;
     uint32_t batch_slots;
+    uint32_t batch_steps;        /* steps per captured replay (1..64)                           */
+    uint32_t pad;
+    uint64_t batch_step_budget;  /* candidates of one step over all running queries            */
 } gfsm_tuning;
 gfsm_status gfsm_tuning_get(gfsm_ctx *ctx, gfsm_tuning *t);
 gfsm_status gfsm_tuning_set(gfsm_ctx *ctx, const gfsm_tuning *t);

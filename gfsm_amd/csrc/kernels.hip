@@ -211,6 +211,30 @@ DEV uint64_t lookback(const uint64_t *words, uint32_t ep, uint64_t t, uint64_t t
     return sum;
 }
 
+// Tile tickets: tiles are handed out in launch order, so a tile only waits on
+// tiles already owned by running workgroups.  Each workgroup holds one
+// prefetched ticket (in LDS) so the atomic's latency overlaps work.  The
+// prefetch is issued once the current tile no longer waits on anyone --
+// right after its look-back, or at tile start for kinds without one -- so a
+// claimed tile is never left idle behind a whole tile's work: a tile's
+// predecessors are always running or about to start, and the look-back
+// does not queue behind them.
+DEV void tickets_init(const ScanWS &ws, uint32_t *s_next)
+{
+    if (threadIdx.x == 0) *s_next = atomicAdd(ws.ctl + 0, 1u);
+}
+
+DEV uint64_t tickets_take(uint32_t *s_next)
+{
+    __syncthreads();
+    return *s_next;  // overwritten only after a later barrier (ticket_prefetch)
+}
+
+DEV void ticket_prefetch(const ScanWS &ws, uint32_t *s_next)
+{
+    if (threadIdx.x == 0) *s_next = atomicAdd(ws.ctl + 0, 1u);
+}
+
 template <uint32_t I>
 struct ScanSmem {
     uint32_t wk[I][kWaves];
@@ -223,7 +247,7 @@ struct ScanSmem {
 // learn its exclusive prefix: wave 0 chains with the preceding tiles of the
 // scan that starts at tile t0.
 template <uint32_t I>
-DEV void tile_prefix(uint64_t t, uint64_t t0, const ScanWS &ws, uint32_t ep, ScanSmem<I> &sm)
+DEV void tile_prefix(uint64_t t, uint64_t t0, const ScanWS &ws, uint32_t ep, ScanSmem<I> &sm, uint32_t *s_next)
 {
     const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
     if (wave == 0) {
@@ -254,6 +278,7 @@ DEV void tile_prefix(uint64_t t, uint64_t t0, const ScanWS &ws, uint32_t ep, Sca
         }
     }
     __syncthreads();
+    ticket_prefetch(ws, s_next);  // this tile waits on nobody any more
 }
 
 // Tile-wide exclusive prefix of (keep, cnt) in candidate order (item k of
@@ -262,8 +287,8 @@ DEV void tile_prefix(uint64_t t, uint64_t t0, const ScanWS &ws, uint32_t ep, Sca
 // through this tile.
 template <uint32_t I>
 DEV void tile_scan(const bool (&keep)[I], const uint32_t (&cnt)[I], uint64_t t, uint64_t t0, const ScanWS &ws,
-                   uint32_t ep, ScanSmem<I> &sm, uint64_t (&my_k)[I], uint64_t (&my_c)[I], uint64_t &run_k,
-                   uint64_t &run_c)
+                   uint32_t ep, ScanSmem<I> &sm, uint32_t *s_next, uint64_t (&my_k)[I], uint64_t (&my_c)[I],
+                   uint64_t &run_k, uint64_t &run_c)
 {
     const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
     uint32_t kpre[I], cpre[I];
@@ -279,7 +304,7 @@ DEV void tile_scan(const bool (&keep)[I], const uint32_t (&cnt)[I], uint64_t t, 
         }
     }
     __syncthreads();
-    tile_prefix<I>(t, t0, ws, ep, sm);
+    tile_prefix<I>(t, t0, ws, ep, sm, s_next);
     uint64_t bk = sm.ex[0], bc = sm.ex[1];
 #pragma unroll
     for (uint32_t k = 0; k < I; k++) {
@@ -303,24 +328,6 @@ DEV void tile_scan(const bool (&keep)[I], const uint32_t (&cnt)[I], uint64_t t, 
 DEV uint32_t read_epoch(const ScanWS &ws)
 {
     return __hip_atomic_load(ws.ctl + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Tile tickets: tiles are handed out in launch order, so a tile only waits on
-// tiles already owned by running workgroups (a prefetched ticket is always
-// processed next by its owner, after its current tile).  Thread 0 keeps one
-// ticket in flight so the atomic's latency overlaps the current tile.
-DEV void tickets_init(const ScanWS &ws, uint32_t &next)
-{
-    if (threadIdx.x == 0) next = atomicAdd(ws.ctl + 0, 1u);
-}
-
-DEV uint64_t tickets_take(const ScanWS &ws, uint32_t &next, uint64_t *s_tile, uint64_t ntiles)
-{
-    if (threadIdx.x == 0) *s_tile = next;
-    __syncthreads();
-    const uint64_t t = *s_tile;
-    if (threadIdx.x == 0 && t < ntiles) next = atomicAdd(ws.ctl + 0, 1u);
-    return t;
 }
 
 // The last workgroup to finish resets the ticket and advances the epoch.
@@ -373,6 +380,7 @@ struct TileShared {
     uint64_t node[2];
     uint64_t red[3][kWaves];
     uint32_t slot;
+    uint32_t next;  // prefetched tile ticket
 };
 
 // ---------------------------------------------------------------------------
@@ -383,7 +391,7 @@ struct TileShared {
 // gt0): keep candidates that carry every self-loop relation of the root.
 template <uint32_t I>
 DEV void root_tile(const DevGraphView &g, const RootIO &io, uint32_t nchk, const uint32_t *chk_rel, uint64_t lt,
-                   uint64_t gt, uint64_t gt0, const ScanWS &ws, uint32_t ep, ScanSmem<I> &sm)
+                   uint64_t gt, uint64_t gt0, const ScanWS &ws, uint32_t ep, ScanSmem<I> &sm, uint32_t *s_next)
 {
     bool keep[I];
     uint32_t v[I], cnt[I];
@@ -408,7 +416,7 @@ DEV void root_tile(const DevGraphView &g, const RootIO &io, uint32_t nchk, const
     for (uint32_t k = 0; k < I; k++)
         if (!keep[k]) cnt[k] = 0;
     uint64_t kk[I], off[I], rk, rc;
-    tile_scan<I>(keep, cnt, gt, gt0, ws, ep, sm, kk, off, rk, rc);
+    tile_scan<I>(keep, cnt, gt, gt0, ws, ep, sm, s_next, kk, off, rk, rc);
 #pragma unroll
     for (uint32_t k = 0; k < I; k++) {
         if (keep[k] && kk[k] < io.cap) {
@@ -430,12 +438,11 @@ __global__ __launch_bounds__(kTile) void k_root(RootArgs a)
 {
     __shared__ TileShared<I> sh;
     const uint32_t ep = read_epoch(a.ws);
-    uint32_t next = 0;
-    tickets_init(a.ws, next);
+    tickets_init(a.ws, &sh.next);
     for (;;) {
-        const uint64_t t = tickets_take(a.ws, next, &sh.tile, a.io.ntiles);
+        const uint64_t t = tickets_take(&sh.next);
         if (t >= a.io.ntiles) break;
-        root_tile<I>(a.g, a.io, a.nchk, a.chk_rel, t, t, 0, a.ws, ep, sh.sm);
+        root_tile<I>(a.g, a.io, a.nchk, a.chk_rel, t, t, 0, a.ws, ep, sh.sm, &sh.next);
     }
     finish_launch(a.ws);
 }
@@ -496,6 +503,7 @@ DEV void expand_tile(const DevGraphView &g, const DepthPlanDev &dp, const uint32
         if (sub == 0) sh.node[tid >> 5] = i;
     }
     __syncthreads();
+    if (KIND == EXPAND_LAST_COUNT) ticket_prefetch(ws, &sh.next);  // no look-back in this kind
     const uint64_t ilo = sh.node[0];
     const uint32_t nf = (uint32_t)(sh.node[1] - ilo + 1);
     const uint32_t nchk = dp.nchk;
@@ -639,7 +647,7 @@ DEV void expand_tile(const DevGraphView &g, const DepthPlanDev &dp, const uint32
             }
         }
         __syncthreads();
-        tile_prefix<I>(gt, gt0, ws, ep, sm);
+        tile_prefix<I>(gt, gt0, ws, ep, sm, &sh.next);
         uint64_t bk = sm.ex[0], bc = sm.ex[1];
         const uint32_t *rows = staged ? s_rows : in_rows;
 #pragma unroll
@@ -705,10 +713,9 @@ __global__ __launch_bounds__(kTile) void k_expand(ExpandArgs a)
     __shared__ TileShared<I> sh;
     const uint32_t ep = read_epoch(a.ws);
     uint64_t acc_cnt = 0, acc_h0 = 0, acc_h1 = 0;
-    uint32_t next = 0;
-    tickets_init(a.ws, next);
+    tickets_init(a.ws, &sh.next);
     for (;;) {
-        const uint64_t t = tickets_take(a.ws, next, &sh.tile, a.io.ntiles);
+        const uint64_t t = tickets_take(&sh.next);
         if (t >= a.io.ntiles) break;
         expand_tile<KIND, I>(a.g, a.dp, a.col, a.dcol, a.io, t, t, 0, a.ws, ep, a.pool_words, sh, smem, acc_cnt,
                              acc_h0, acc_h1);
@@ -1010,10 +1017,9 @@ __global__ __launch_bounds__(kTile) void k_batch_step(BatchCfg cfg)
     __shared__ TileShared<I> sh;
     const uint64_t total = cfg.ctl->total_tiles;
     const uint32_t ep = read_epoch(cfg.ws);
-    uint32_t next = 0;
-    tickets_init(cfg.ws, next);
+    tickets_init(cfg.ws, &sh.next);
     for (;;) {
-        const uint64_t t = tickets_take(cfg.ws, next, &sh.tile, total);
+        const uint64_t t = tickets_take(&sh.next);
         if (t >= total) break;
         if (threadIdx.x < 64) {
             const uint32_t s = slot_search(cfg.tile_start, cfg.nslots, t);
@@ -1026,7 +1032,7 @@ __global__ __launch_bounds__(kTile) void k_batch_step(BatchCfg cfg)
         const uint64_t t0 = B.tile0;
         const QueryPlanDev *plan = B.plan;
         if (step == STEP_ROOT) {
-            root_tile<I>(cfg.g, B.rio, plan->nroot_chk, plan->root_chk_rel, t - t0, t, t0, cfg.ws, ep, sh.sm);
+            root_tile<I>(cfg.g, B.rio, plan->nroot_chk, plan->root_chk_rel, t - t0, t, t0, cfg.ws, ep, sh.sm, &sh.next);
         } else if (step == EXPAND_MID) {
             uint64_t c = 0, h0 = 0, h1 = 0;
             expand_tile<EXPAND_MID, I>(cfg.g, plan->depth[B.d], plan->col, plan->dcol, B.io, t - t0, t, t0, cfg.ws,
@@ -1051,10 +1057,9 @@ __global__ __launch_bounds__(kTile) void k_scan_test(const uint32_t *vals, uint6
     constexpr uint32_t I = kScanTestItems;
     __shared__ TileShared<I> sh;
     const uint32_t ep = read_epoch(ws);
-    uint32_t next = 0;
-    tickets_init(ws, next);
+    tickets_init(ws, &sh.next);
     for (;;) {
-        const uint64_t t = tickets_take(ws, next, &sh.tile, ntiles);
+        const uint64_t t = tickets_take(&sh.next);
         if (t >= ntiles) break;
         bool keep[I];
         uint32_t v[I];
@@ -1065,7 +1070,7 @@ __global__ __launch_bounds__(kTile) void k_scan_test(const uint32_t *vals, uint6
             keep[k] = v[k] != 0;
         }
         uint64_t kx[I], off[I], rk, rc;
-        tile_scan<I>(keep, v, t, 0, ws, ep, sh.sm, kx, off, rk, rc);
+        tile_scan<I>(keep, v, t, 0, ws, ep, sh.sm, &sh.next, kx, off, rk, rc);
 #pragma unroll
         for (uint32_t k = 0; k < I; k++) {
             const uint64_t i = t * (kTile * I) + k * kTile + threadIdx.x;

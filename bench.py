@@ -123,9 +123,8 @@ def batch_section():
     ml = [l for _, l in caps]
     ctx = G.Context(0, mem_budget=64 << 30)
     ctx.upload(data)
-    # autotune the batch shapes on the uncapped queries among the first 64
-    sample = [i for i in range(64) if ml[i] == 0]
-    tuned = ctx.autotune(data, [qs[i] for i in sample], windows=[wins[i] for i in sample], which=G.TUNE_BATCH)
+    # autotune the batch shapes on the first 64 queries (with their windows and caps)
+    tuned = ctx.autotune(data, qs[:64], windows=wins[:64], caps=mc[:64], depth_caps=ml[:64], which=G.TUNE_BATCH)
     ctx.match_batch(data, qs[:16], windows=wins[:16], caps=mc[:16], depth_caps=ml[:16])  # engine set-up
     for i in range(16):  # single-query arena set-up, likewise outside the timing
         ctx.match(data, qs[i], root_lo=wins[i][0], root_hi=wins[i][1], max_cand=mc[i], max_depth_cand=ml[i],
@@ -171,6 +170,9 @@ def serve_section(n):
     cap, last = W.CONFIG5_CAP, W.CONFIG5_DEPTH_CAP
     ctx = G.Context(0, mem_budget=16 << 30)
     ctx.upload(data)
+    # batch shapes tuned on the first 256 queries of the stream (the server runs on the batch engine)
+    tuned = ctx.autotune(data, qs[:256], windows=wins[:256], caps=[cap] * 256, depth_caps=[last] * 256,
+                         which=G.TUNE_BATCH)
     for q, (lo, hi) in list(zip(qs, wins))[:32]:  # warm the single-query path
         ctx.match(data, q, root_lo=lo, root_hi=hi, max_cand=cap, max_depth_cand=last, allow_partial=True)
     svc, ser = [], []
@@ -232,6 +234,7 @@ def serve_section(n):
         "equal_saturated": f"{eq_sat}/{n}",
         "equal_open_loop": f"{eq_ol}/{n}",
         "embeddings": int(sum(m.count for m in sat)),
+        "tuned_shape": {k: tuned[k] for k in ("batch_items", "batch_pool_words", "batch_wgs_per_cu")},
     }
 
 

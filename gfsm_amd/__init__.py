@@ -131,7 +131,7 @@ def lib():
         "gfsm_match_batch": [vp, vp, P(vp), u32, P(MatchOpts), u32, P(vp)],
         "gfsm_tuning_get": [vp, P(Tuning)],
         "gfsm_tuning_set": [vp, P(Tuning)],
-        "gfsm_autotune": [vp, vp, P(vp), pu32, pu32, u32, u32, P(Tuning)],
+        "gfsm_autotune": [vp, vp, P(vp), u32, P(MatchOpts), u32, u32, P(Tuning)],
         "gfsm_server_start": [vp, vp, P(ServeOpts), P(vp)],
         "gfsm_server_submit": [vp, vp, P(MatchOpts), P(u64)],
         "gfsm_server_poll": [vp, u64, P(vp)],
@@ -421,16 +421,18 @@ class Context:
                 setattr(t, k, v)
         _check(lib().gfsm_tuning_set(self._h, C.byref(t)), "tuning_set")
 
-    def autotune(self, data: Graph, queries, windows=None, which=3) -> dict:
-        """Time the sample under each candidate launch shape; keep the fastest."""
+    def autotune(self, data: Graph, queries, windows=None, caps=None, depth_caps=None, which=3) -> dict:
+        """Time the sample (per-query windows and caps optional) under each
+        candidate launch shape; keep the fastest."""
         n = len(queries)
         hs = (C.c_void_p * n)(*[q._h.value for q in queries])
-        lo = hi = None
-        if windows is not None:
-            lo = (C.c_uint32 * n)(*[w[0] for w in windows])
-            hi = (C.c_uint32 * n)(*[w[1] for w in windows])
+        wins = windows if windows is not None else [(0, 0)] * n
+        cps = caps if caps is not None else [0] * n
+        dcs = depth_caps if depth_caps is not None else [0] * n
+        os_ = (MatchOpts * n)(*[_opts(root_lo=lo, root_hi=hi, max_cand=c, max_depth_cand=d)
+                                for (lo, hi), c, d in zip(wins, cps, dcs)])
         t = Tuning()
-        _check(lib().gfsm_autotune(self._h, data._h, hs, lo, hi, n, which, C.byref(t)), "autotune")
+        _check(lib().gfsm_autotune(self._h, data._h, hs, n, os_, n, which, C.byref(t)), "autotune")
         return t.as_dict()
 
 

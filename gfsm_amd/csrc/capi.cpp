@@ -394,21 +394,22 @@ gfsm_status gfsm_tuning_set(gfsm_ctx *ctx, const gfsm_tuning *t)
     GUARD_END
 }
 
-gfsm_status gfsm_autotune(gfsm_ctx *ctx, const gfsm_graph *data, const gfsm_graph *const *queries,
-                          const uint32_t *root_lo, const uint32_t *root_hi, uint32_t n, uint32_t which,
-                          gfsm_tuning *t)
+gfsm_status gfsm_autotune(gfsm_ctx *ctx, const gfsm_graph *data, const gfsm_graph *const *queries, uint32_t n,
+                          const gfsm_match_opts *opts, uint32_t nopts, uint32_t which, gfsm_tuning *t)
 {
     GUARD_BEGIN
     if (!ctx || !data || !n || !queries) return fail(Status::err(GFSM_ERR_ARG, "null argument"));
+    if (opts && nopts != 1 && nopts != n) return fail(Status::err(GFSM_ERR_ARG, "nopts must be 1 or n"));
     std::vector<const HostGraph *> qs(n);
-    std::vector<uint32_t> lo(n, 0), hi(n, 0);
+    std::vector<gfsm_match_opts> os(n);
     for (uint32_t i = 0; i < n; i++) {
         if (!queries[i]) return fail(Status::err(GFSM_ERR_ARG, "null query"));
         qs[i] = queries[i]->g;
-        if (root_lo) lo[i] = root_lo[i];
-        if (root_hi) hi[i] = root_hi[i];
+        if (opts) os[i] = opts[nopts == 1 ? 0 : i];
+        os[i].flags &= ~(GFSM_ROWS | GFSM_SORTED | GFSM_PROFILE);  // count mode
+        os[i].tile = 0;
     }
-    Status st = autotune(*ctx->c, *data->g, qs, lo, hi, which);
+    Status st = autotune(*ctx->c, *data->g, qs, os, which);
     if (!st.good()) return fail(st);
     if (t) return gfsm_tuning_get(ctx, t);
     return ok();

@@ -54,7 +54,7 @@ struct Arena {
 struct Tuning {
     uint8_t items[3][GFSM_MAXQ];
     uint32_t pool_words = kPoolWords;
-    uint32_t wgs_per_cu = 8;
+    uint32_t wgs_per_cu = 4;
     uint32_t batch_items = 8;
     uint32_t batch_pool_words = 14336;
     uint32_t batch_wgs_per_cu = 4;

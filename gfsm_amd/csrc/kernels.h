@@ -123,7 +123,7 @@ constexpr uint32_t kTile = 256;     // threads per workgroup
 constexpr uint32_t kItems = 8;      // default candidates per thread per tile
 constexpr uint32_t kRootItems = 4;  // root level (single-query path)
 constexpr uint32_t kRootTile = kTile * kRootItems;
-constexpr uint32_t kPoolWords = 8192;  // default staging pool (32 KB)
+constexpr uint32_t kPoolWords = 7680;  // default staging pool (30 KB: four workgroups fit a CU's LDS)
 
 // Dynamic LDS bytes of the expand and batch-step kernels for `items`
 // candidates per thread and a staging pool of `pool_words` words.

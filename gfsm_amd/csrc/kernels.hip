@@ -706,8 +706,11 @@ DEV void expand_tile(const DevGraphView &g, const DepthPlanDev &dp, const uint32
     }
 }
 
+// Four waves per SIMD (at most 128 VGPRs): the kernel is bound by the latency
+// of dependent loads, and the extra resident workgroup hides more of it than
+// the few registers spilled at 8 candidates per thread cost (measured).
 template <uint32_t KIND, uint32_t I>
-__global__ __launch_bounds__(kTile) void k_expand(ExpandArgs a)
+__global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(4))) void k_expand(ExpandArgs a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ TileShared<I> sh;

@@ -132,7 +132,7 @@ Status match_batch_shaped(Ctx &ctx, const HostGraph &data, const std::vector<con
 
 // Shape search over the batch engine and the single-query path.
 Status autotune(Ctx &ctx, const HostGraph &data, const std::vector<const HostGraph *> &queries,
-                const std::vector<uint32_t> &root_lo, const std::vector<uint32_t> &root_hi, uint32_t which);
+                const std::vector<gfsm_match_opts> &opts, uint32_t which);
 
 struct Server;
 Status server_start(Ctx &ctx, const HostGraph &data, const gfsm_serve_opts &opts, Server *&out);

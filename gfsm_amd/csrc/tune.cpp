@@ -1,6 +1,7 @@
 // tune.cpp -- the autotuner (no reference counterpart: the reference
 // launches fixed shapes, HnGPU.cu:45-47).  It times a caller-chosen sample
-// under each candidate launch shape and keeps the fastest in the context:
+// (queries with their windows and caps) under each candidate launch shape
+// and keeps the fastest in the context:
 //   single-query path: candidates per thread x LDS staging pool x
 //                      persistent workgroups per CU,
 //   batch path:        the same three for the batch step kernel.
@@ -17,20 +18,19 @@ namespace {
 struct Sample {
     const HostGraph &data;
     const std::vector<const HostGraph *> &queries;
-    const std::vector<uint32_t> &lo, &hi;
+    const std::vector<gfsm_match_opts> &opts;  // one per query (windows, caps)
 };
+
+bool ok_or_capped(gfsm_status s) { return s == GFSM_OK || s == GFSM_ERR_CAP; }
 
 // Device milliseconds of the sample on the single-query path.
 Status time_plain(Ctx &ctx, const Sample &s, double &ms)
 {
     ms = 0;
     for (size_t i = 0; i < s.queries.size(); i++) {
-        gfsm_match_opts o{};
-        o.root_lo = s.lo[i];
-        o.root_hi = s.hi[i];
         Result r;
-        Status st = match_plain(ctx, s.data, *s.queries[i], o, r);
-        if (!st.good()) return st;
+        Status st = match_plain(ctx, s.data, *s.queries[i], s.opts[i], r);
+        if (!st.good() && st.code != GFSM_ERR_CAP) return st;
         ms += r.stats.device_ms;
     }
     return Status::ok();
@@ -38,29 +38,24 @@ Status time_plain(Ctx &ctx, const Sample &s, double &ms)
 
 Status time_batch(Ctx &ctx, const Sample &s, const BatchShape &shape, double &ms)
 {
-    std::vector<gfsm_match_opts> opts(s.queries.size());
-    for (size_t i = 0; i < opts.size(); i++) {
-        opts[i].root_lo = s.lo[i];
-        opts[i].root_hi = s.hi[i];
-    }
     std::vector<Result> out;
     ms = 0;
-    Status st = match_batch_shaped(ctx, s.data, s.queries, opts, out, shape, &ms);
+    Status st = match_batch_shaped(ctx, s.data, s.queries, s.opts, out, shape, &ms);
     if (!st.good()) return st;
     for (auto &r : out)
-        if (r.status != GFSM_OK) return Status::err(r.status, "autotune sample query failed");
+        if (!ok_or_capped(r.status)) return Status::err(r.status, "autotune sample query failed");
     return Status::ok();
 }
 
 }  // namespace
 
 Status autotune(Ctx &ctx, const HostGraph &data, const std::vector<const HostGraph *> &queries,
-                const std::vector<uint32_t> &root_lo, const std::vector<uint32_t> &root_hi, uint32_t which)
+                const std::vector<gfsm_match_opts> &opts, uint32_t which)
 {
-    if (queries.empty()) return Status::err(GFSM_ERR_ARG, "empty autotune sample");
-    Sample s{data, queries, root_lo, root_hi};
+    if (queries.empty() || opts.size() != queries.size()) return Status::err(GFSM_ERR_ARG, "bad autotune sample");
+    Sample s{data, queries, opts};
     const uint32_t item_set[] = {2, 4, 8};
-    const uint32_t pool_set[] = {4096, 8192, 14336};
+    const uint32_t pool_set[] = {4096, 7680, 14336};
     const uint32_t wgs_set[] = {4, 8};
     const bool verbose = std::getenv("GFSM_TUNE_VERBOSE") != nullptr;
 

@@ -241,16 +241,16 @@ typedef struct gfsm_tuning {
 gfsm_status gfsm_tuning_get(gfsm_ctx *ctx, gfsm_tuning *t);
 gfsm_status gfsm_tuning_set(gfsm_ctx *ctx, const gfsm_tuning *t);
 
-/* Autotuner: runs the sample -- n queries, each over its root window
- * [root_lo[i], root_hi[i]) (NULL arrays = whole range), count mode -- under
- * each candidate shape, keeps the fastest in ctx and returns it in *t.
- * which: 1 = single-query shapes, 2 = batch shapes, 3 = both.  The sample
- * should take tens of milliseconds per shape. */
+/* Autotuner: runs the sample -- n queries with their options (nopts = 1:
+ * the same for all, or n: root windows and caps per query; NULL = whole
+ * range, no caps), count mode -- under each candidate shape, keeps the
+ * fastest in ctx and returns it in *t.  which: 1 = single-query shapes, 2 =
+ * batch shapes, 3 = both.  The sample should take tens of milliseconds per
+ * shape. */
 #define GFSM_TUNE_PLAIN 1u
 #define GFSM_TUNE_BATCH 2u
-gfsm_status gfsm_autotune(gfsm_ctx *ctx, const gfsm_graph *data, const gfsm_graph *const *queries,
-                          const uint32_t *root_lo, const uint32_t *root_hi, uint32_t n, uint32_t which,
-                          gfsm_tuning *t);
+gfsm_status gfsm_autotune(gfsm_ctx *ctx, const gfsm_graph *data, const gfsm_graph *const *queries, uint32_t n,
+                          const gfsm_match_opts *opts, uint32_t nopts, uint32_t which, gfsm_tuning *t);
 
 /* ---- seeded workload generators (undirected graphs as symmetric entry lists) */
 #define GFSM_WALK_DENSE 0u   /* keep every induced edge among the walked vertices */

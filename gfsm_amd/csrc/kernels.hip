@@ -33,6 +33,7 @@ namespace gfsm {
 #define DEV __device__ __forceinline__
 
 constexpr uint32_t kWaves = kTile / 64;
+constexpr uint32_t kNodeLk = 4;  // node-setup lookups a thread keeps in flight
 
 // ---------------------------------------------------------------------------
 // wave64 primitives
@@ -94,7 +95,7 @@ DEV uint32_t seg_hash(uint32_t x, uint32_t rel)
 // or act[k] is false: linear probing in the segment hash table, the I
 // lookups issuing their probes together.
 template <uint32_t I>
-DEV void seg_lookup_n(const DevGraphView &g, const uint32_t (&x)[I], uint32_t rel, const bool (&act)[I],
+DEV void seg_lookup_r(const DevGraphView &g, const uint32_t (&x)[I], const uint32_t (&rel)[I], const bool (&act)[I],
                       uint2 (&seg)[I])
 {
     uint32_t p[I];
@@ -102,7 +103,7 @@ DEV void seg_lookup_n(const DevGraphView &g, const uint32_t (&x)[I], uint32_t re
 #pragma unroll
     for (uint32_t k = 0; k < I; k++) {
         pend[k] = act[k];
-        p[k] = seg_hash(x[k], rel) & g.seg_mask;
+        p[k] = seg_hash(x[k], rel[k]) & g.seg_mask;
         seg[k] = make_uint2(0u, 0u);
     }
     for (;;) {
@@ -116,7 +117,7 @@ DEV void seg_lookup_n(const DevGraphView &g, const uint32_t (&x)[I], uint32_t re
 #pragma unroll
         for (uint32_t k = 0; k < I; k++) {
             if (!pend[k]) continue;
-            if (e[k].x == x[k] && e[k].y == rel) {
+            if (e[k].x == x[k] && e[k].y == rel[k]) {
                 seg[k] = make_uint2(e[k].z, e[k].w);
                 pend[k] = false;
             } else if (e[k].x == kSegEmpty) {
@@ -126,6 +127,17 @@ DEV void seg_lookup_n(const DevGraphView &g, const uint32_t (&x)[I], uint32_t re
             }
         }
     }
+}
+
+// The same I lookups, all with one relation.
+template <uint32_t I>
+DEV void seg_lookup_n(const DevGraphView &g, const uint32_t (&x)[I], uint32_t rel, const bool (&act)[I],
+                      uint2 (&seg)[I])
+{
+    uint32_t r[I];
+#pragma unroll
+    for (uint32_t k = 0; k < I; k++) r[k] = rel;
+    seg_lookup_r<I>(g, x, r, act, seg);
 }
 
 // keep[k] &= (w[k] is in the sorted slice nbr[seg[k].x, seg[k].y)): the last
@@ -518,32 +530,47 @@ DEV void expand_tile(const DevGraphView &g, const DepthPlanDev &dp, const uint32
 
     if (staged)
         for (uint32_t u = tid; u < nf * d; u += T) s_rows[u] = in_rows[u];
-    // Per frontier node: start bit, generator base, staged segments.
+    // Per frontier node: start bit, generator base.
     for (uint32_t j = tid; j < nf; j += T) {
         const uint64_t i = ilo + j;
         const uint64_t o = io.in_off[i];
         const uint32_t first = (o > c0) ? (uint32_t)(o - c0) : 0u;
         atomicOr((unsigned long long *)&s_bits[first >> 6], 1ull << (first & 63));
         s_seg[j] = io.in_gbeg[i] + (uint32_t)(c0 - o);
-        if (stage_meta) {
-            const uint32_t *av = in_rows + (uint64_t)j * d;
-            for (uint32_t r = 0; r < nchk; r++) {
-                const uint32_t dist = dp.chk_dist[r];
-                if (dist) {
-                    uint32_t x[1] = {av[d - dist]};
-                    bool act[1] = {true};
-                    uint2 sg[1];
-                    seg_lookup_n<1>(g, x, dp.chk_rel[r], act, sg);
-                    s_chk[r * nf + j] = sg[0];
+    }
+    // Staged segments: every (node, lookup) pair of the tile -- the node's
+    // checked ancestors and its next generator -- striped over the threads
+    // and resolved kNodeLk at a time in lockstep, so a node's lookups do not
+    // run one after another.
+    if (stage_meta) {
+        constexpr uint32_t L = kNodeLk;
+        const uint32_t nl = nchk + need_nxt;
+        const uint32_t total = nf * nl;
+        for (uint32_t u0 = tid; u0 < total; u0 += T * L) {
+            uint32_t x[L], rel[L], dst[L];
+            bool act[L];
+#pragma unroll
+            for (uint32_t k = 0; k < L; k++) {
+                const uint32_t u = u0 + k * T;
+                act[k] = false;
+                x[k] = rel[k] = dst[k] = 0;
+                if (u < total) {
+                    const uint32_t j = u / nl, r = u - j * nl;
+                    // row column of the vertex whose segment is looked up
+                    // (a check's ancestor at distance chk_dist, or the next
+                    // generator's at distance nxt_dist - 1 from this depth)
+                    act[k] = (r < nchk) ? dp.chk_dist[r] != 0 : true;  // self checks: per candidate
+                    const uint32_t col = (r < nchk) ? d - dp.chk_dist[r] : d + 1 - dp.nxt_dist;
+                    rel[k] = (r < nchk) ? dp.chk_rel[r] : dp.nxt_rel;
+                    dst[k] = (r < nchk) ? r * nf + j : nchk * nf + j;  // s_chk[r][j] or s_nxt[j]
+                    if (act[k]) x[k] = in_rows[(uint64_t)j * d + col];
                 }
             }
-            if (need_nxt) {
-                uint32_t x[1] = {av[d + 1 - dp.nxt_dist]};
-                bool act[1] = {true};
-                uint2 sg[1];
-                seg_lookup_n<1>(g, x, dp.nxt_rel, act, sg);
-                s_nxt[j] = sg[0];
-            }
+            uint2 sg[L];
+            seg_lookup_r<L>(g, x, rel, act, sg);
+#pragma unroll
+            for (uint32_t k = 0; k < L; k++)
+                if (act[k]) s_chk[dst[k]] = sg[k];
         }
     }
     __syncthreads();

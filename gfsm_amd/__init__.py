@@ -111,6 +111,8 @@ def lib():
         "gfsm_graph_load_tve": [C.c_char_p, u32, P(vp)],
         "gfsm_graph_from_edges": [u32, pu32, u64, pu32, pu32, pu32, u32, P(vp)],
         "gfsm_graph_write_tve": [vp, C.c_char_p],
+        "gfsm_graph_save_bin": [vp, C.c_char_p],
+        "gfsm_graph_load_bin": [C.c_char_p, P(vp)],
         "gfsm_graph_edges": [vp, P(u32), P(u64), P(pu32), P(pu32), P(pu32), P(pu32)],
         "gfsm_graph_layout": [vp, P(u32), P(u64), P(u64), P(pu32), P(pu32), P(pu32)],
         "gfsm_graph_info": [vp, P(u32), P(u64), P(u32), P(u32)],
@@ -237,6 +239,14 @@ class Graph:
 
     def write(self, path):
         _check(lib().gfsm_graph_write_tve(self._h, str(path).encode()), "write")
+
+    def save_bin(self, path):
+        """Binary image of the built graph (input arrays + layout)."""
+        _check(lib().gfsm_graph_save_bin(self._h, str(path).encode()), "save_bin")
+
+    @classmethod
+    def load_bin(cls, path):
+        return cls._make(lib().gfsm_graph_load_bin, str(path).encode(), what=f"load_bin {path}")
 
     def info(self):
         nv, ne, md, mode = C.c_uint32(), C.c_uint64(), C.c_uint32(), C.c_uint32()

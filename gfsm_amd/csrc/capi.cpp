@@ -93,6 +93,28 @@ gfsm_status gfsm_graph_write_tve(const gfsm_graph *g, const char *path)
     GUARD_END
 }
 
+gfsm_status gfsm_graph_save_bin(const gfsm_graph *g, const char *path)
+{
+    GUARD_BEGIN
+    if (!g || !path) return fail(Status::err(GFSM_ERR_ARG, "null argument"));
+    Status st = save_bin(*g->g, path);
+    return st.good() ? ok() : fail(st);
+    GUARD_END
+}
+
+gfsm_status gfsm_graph_load_bin(const char *path, gfsm_graph **g)
+{
+    GUARD_BEGIN
+    if (!path || !g) return fail(Status::err(GFSM_ERR_ARG, "null argument"));
+    *g = nullptr;
+    HostGraph *h = nullptr;
+    Status st = load_bin(path, h);
+    if (!st.good()) return fail(st);
+    *g = new gfsm_graph{h};
+    return ok();
+    GUARD_END
+}
+
 gfsm_status gfsm_graph_edges(const gfsm_graph *g, uint32_t *nv, uint64_t *ne, const uint32_t **vlabel,
                              const uint32_t **src, const uint32_t **dst, const uint32_t **elabel)
 {

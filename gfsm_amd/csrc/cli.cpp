@@ -2,6 +2,11 @@
 // (main.cu:56-63) over libgfsm.  Prints the reference's CSV row
 // (mainscripts/runLabel.sh:19 column order) and exits with its codes
 // (232/233/234 for input errors, 1 for query errors).
+// --cache PATH keeps a binary image of the data graph: a later run whose
+// image is newer than the data file and of the same mode loads it instead of
+// parsing the text and building the layout again.
+#include <sys/stat.h>
+
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -21,6 +26,29 @@ static std::string base_name(const std::string &p)
     return k == std::string::npos ? p : p.substr(k + 1);
 }
 
+// Data graph through the binary image at `cache` when that image is usable,
+// else from the text (writing the image for next time; a failed write only
+// warns).
+static gfsm_status load_data(const char *path, const char *cache, uint32_t mode, gfsm_graph **d)
+{
+    if (cache) {
+        struct stat ts, cs;
+        auto ns = [](const struct stat &x) { return (long long)x.st_mtim.tv_sec * 1000000000ll + x.st_mtim.tv_nsec; };
+        if (stat(cache, &cs) == 0 && stat(path, &ts) == 0 && ns(cs) >= ns(ts) &&
+            gfsm_graph_load_bin(cache, d) == GFSM_OK) {
+            uint32_t nv, md, m;
+            uint64_t ne;
+            if (gfsm_graph_info(*d, &nv, &ne, &md, &m) == GFSM_OK && m == mode) return GFSM_OK;
+            gfsm_graph_free(*d);
+            *d = nullptr;
+        }
+    }
+    gfsm_status st = gfsm_graph_load_tve(path, mode, d);
+    if (st == GFSM_OK && cache && gfsm_graph_save_bin(*d, cache) != GFSM_OK)
+        std::fprintf(stderr, "warning: could not write %s: %s\n", cache, gfsm_last_error());
+    return st;
+}
+
 static int die(gfsm_status st)
 {
     std::fprintf(stderr, "\nGFSM error %d: %s\n", st, gfsm_last_error());
@@ -32,12 +60,12 @@ int main(int argc, char **argv)
     if (argc < 3) {
         std::fprintf(stderr,
                      "usage: %s <query.g> <data.g> [--mode ref|vlabel] [--out output.fs] [--sorted]\n"
-                     "          [--mem-budget BYTES] [--device N] [--digest]\n",
+                     "          [--mem-budget BYTES] [--device N] [--digest] [--cache data.bin]\n",
                      argv[0]);
         return 2;
     }
     uint32_t mode = GFSM_MODE_REF;
-    const char *out = nullptr;
+    const char *out = nullptr, *cache = nullptr;
     uint64_t budget = 0;
     int device = 0;
     uint32_t flags = 0;
@@ -50,6 +78,8 @@ int main(int argc, char **argv)
             budget = std::strtoull(argv[++i], nullptr, 10);
         } else if (!std::strcmp(argv[i], "--device") && i + 1 < argc) {
             device = std::atoi(argv[++i]);
+        } else if (!std::strcmp(argv[i], "--cache") && i + 1 < argc) {
+            cache = argv[++i];
         } else if (!std::strcmp(argv[i], "--digest")) {
             flags |= GFSM_DIGEST;
         } else if (!std::strcmp(argv[i], "--sorted")) {
@@ -67,7 +97,7 @@ int main(int argc, char **argv)
     if (st) return die(st);
     const double q_parse = now_s() - t;
     t = now_s();
-    st = gfsm_graph_load_tve(argv[2], mode, &d);
+    st = load_data(argv[2], cache, mode, &d);
     if (st) return die(st);
     const double d_parse = now_s() - t;
     uint32_t qn = 0;

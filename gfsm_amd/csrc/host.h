@@ -77,6 +77,9 @@ Status build_graph(uint32_t nv, const uint32_t *vlabel, uint64_t ne, const uint3
                    const uint32_t *dst, const uint32_t *elabel, uint32_t mode, HostGraph *&out);
 Status load_tve(const char *path, uint32_t mode, HostGraph *&out);
 Status write_tve(const HostGraph &g, const char *path);
+// Versioned binary image of a built graph (input arrays + layout), checked on load.
+Status save_bin(const HostGraph &g, const char *path);
+Status load_bin(const char *path, HostGraph *&out);
 
 // Compiled query (matching order + per-depth requirements).
 struct Plan {

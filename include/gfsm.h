@@ -12,6 +12,8 @@
  *   gfsm_open / gfsm_close      HnSetup::preinit (HnGPU.cu:847-862) + the device
  *                               reset/warm-up of argTest (main.cu:11-20)
  *   gfsm_graph_load_tve         fileParse (HnHelper.cuh:283-499)
+ *   gfsm_graph_save/load_bin    (no counterpart) binary graph image that skips
+ *                               fileParse's parse + layout on later runs
  *   gfsm_graph_from_edges       fileParse's row staging + CCSR layout
  *                               (HnHelper.cuh:364-478) from in-memory arrays
  *   gfsm_plan_query             HnSetup::preProcessQuery (HnGPU.cu:749-816) +
@@ -126,6 +128,14 @@ gfsm_status gfsm_graph_from_edges(uint32_t nv, const uint32_t *vlabel, uint64_t 
                                   const uint32_t *src, const uint32_t *dst, const uint32_t *elabel,
                                   uint32_t mode, gfsm_graph **g);
 gfsm_status gfsm_graph_write_tve(const gfsm_graph *g, const char *path);
+/* Versioned binary image of a built graph: its input arrays and the
+ * relation-partitioned layout, so a later run skips the text parse and the
+ * layout build that the reference repeats on every start (fileParse,
+ * HnHelper.cuh:283-499, parses and builds its CCSR).  No reference
+ * counterpart.  Loading checks the magic, version, sizes, a payload checksum
+ * and the layout invariants: 232 unreadable file, 233 anything else wrong. */
+gfsm_status gfsm_graph_save_bin(const gfsm_graph *g, const char *path);
+gfsm_status gfsm_graph_load_bin(const char *path, gfsm_graph **g);
 /* Views of the graph's input arrays (valid while g lives). */
 gfsm_status gfsm_graph_edges(const gfsm_graph *g, uint32_t *nv, uint64_t *ne, const uint32_t **vlabel,
                              const uint32_t **src, const uint32_t **dst, const uint32_t **elabel);

@@ -167,3 +167,86 @@ def test_cli_input_errors_need_no_device(tmp_path):
     disc = tmp_path / "disc.g"
     disc.write_text("t # 0\n4 4 1 1\nv 0 1\nv 1 1\nv 2 1\nv 3 1\ne 0 1 1\ne 1 0 1\ne 2 3 1\ne 3 2 1\nt # -1")
     assert subprocess.run([cli, str(disc), q], capture_output=True).returncode == 1
+
+
+@pytest.mark.parametrize("mode", [G.MODE_REF, G.MODE_VLABEL])
+def test_binary_image_round_trip(tmp_path, mode):
+    """A saved image loads back to the same input arrays, layout, relation
+    index (seen through root sets and windows) and oracle result."""
+    for g in (G.Graph.er(400, 6.0, nvlabels=3, nelabels=2, seed=8, mode=mode),
+              G.Graph.rmat(10, 5, nvlabels=4, nelabels=3, seed=9, mode=mode),
+              G.Graph.from_edges(5, [], [], [], mode=mode)):
+        p = tmp_path / "g.bin"
+        g.save_bin(p)
+        h = G.Graph.load_bin(p)
+        assert h.info() == g.info()
+        for a, b in zip(g.edges()[1:], h.edges()[1:]):
+            assert np.array_equal(a, b)
+        for a, b in zip(g.layout(), h.layout()):
+            assert np.array_equal(a, b)
+        if g.info()["entries"] == 0:
+            continue
+        check_layout(h)
+        for seed in range(4):
+            q = G.Graph.walk_query(g, 4, kind=seed % 2, seed=seed)
+            assert np.array_equal(g.roots(q), h.roots(q))
+            assert g.window(q, 300) == h.window(q, 300)
+        q = G.Graph.walk_query(g, 3, seed=11)
+        a = O.solve(to_oracle(g), to_oracle(q), mode=mode, rows=False)
+        b = O.solve(to_oracle(h), to_oracle(q), mode=mode, rows=False)
+        assert (a.count, a.digest) == (b.count, b.digest)
+
+
+def test_binary_image_rejects_damage(tmp_path):
+    g = G.Graph.er(200, 4.0, nvlabels=2, nelabels=2, seed=3, mode=G.MODE_VLABEL)
+    p = tmp_path / "g.bin"
+    g.save_bin(p)
+    raw = bytearray(p.read_bytes())
+    header = 8 + 4 * 4 + 6 * 8
+    cases = {
+        "magic": raw[:0] + b"XFSMBIN1" + raw[8:],
+        "version": raw[:8] + (7).to_bytes(4, "little") + raw[12:],
+        "mode": raw[:12] + (9).to_bytes(4, "little") + raw[16:],
+        "truncated": raw[:-5],
+        "extended": raw + b"\0" * 8,
+        "payload_bit": raw[:header + 40] + bytes([raw[header + 40] ^ 1]) + raw[header + 41:],
+        "last_byte": raw[:-1] + bytes([raw[-1] ^ 0x80]),
+        "short_header": raw[:20],
+    }
+    for name, blob in cases.items():
+        q = tmp_path / f"{name}.bin"
+        q.write_bytes(bytes(blob))
+        with pytest.raises(G.GfsmError) as ei:
+            G.Graph.load_bin(q)
+        assert ei.value.code == 233, name
+    with pytest.raises(G.GfsmError) as ei:
+        G.Graph.load_bin(tmp_path / "missing.bin")
+    assert ei.value.code == 232
+    with pytest.raises(G.GfsmError) as ei:  # a text graph is not an image
+        G.Graph.load_bin(os.path.join(GOLD, "queries", "square.g"))
+    assert ei.value.code == 233
+
+
+def test_cli_cache_uses_image_of_same_mode(tmp_path):
+    """--cache writes the image on the first run; a later run reads it (a
+    damaged image would then fail, so it must be rebuilt when the data file
+    is newer or the mode differs)."""
+    cli = os.path.join(ROOT, "gfsm_amd", "GFSM")
+    q = os.path.join(GOLD, "queries", "square.g")
+    data = tmp_path / "bad.g"
+    data.write_text("t # 0\n3 0 1 1\nv 0 1\nt # -1")  # missing data: fails before any device use
+    cache = tmp_path / "d.bin"
+    r = subprocess.run([cli, q, str(data), "--cache", str(cache)], capture_output=True)
+    assert r.returncode == 234 and not cache.exists()
+    # an image of a valid graph in vlabel mode, newer than the (bad) text
+    G.Graph.er(50, 3.0, seed=1, mode=G.MODE_VLABEL).save_bin(cache)
+    os.utime(data, (1, 1))
+    r = subprocess.run([cli, q, str(data), "--cache", str(cache)], capture_output=True)
+    assert r.returncode == 234  # ref mode asked: image of another mode ignored, text parsed
+    r = subprocess.run([cli, q, str(data), "--cache", str(cache), "--mode", "vlabel", "--device", "99"],
+                       capture_output=True)
+    assert r.returncode != 234  # image used: parsing skipped, so the (absent) device is what fails
+    later = os.stat(cache).st_mtime + 10
+    os.utime(data, (later, later))  # text newer than the image: parsed again
+    r = subprocess.run([cli, q, str(data), "--cache", str(cache), "--mode", "vlabel"], capture_output=True)
+    assert r.returncode == 234

@@ -35,8 +35,9 @@ static double now_ms()
         if (e_ != hipSuccess) return hip_status(e_, #expr);              \
     } while (0)
 
-Status Lane::init(uint64_t max_tiles)
+Status Lane::init(uint64_t max_tiles, bool tile_index)
 {
+    if (tile_index) HIP_TRY(hipMalloc(&tile_node, 2 * max_tiles * sizeof(uint64_t)));
     HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     ws.max_tiles = max_tiles;
     HIP_TRY(hipMalloc(&ws.wk, max_tiles * sizeof(uint64_t)));
@@ -72,6 +73,8 @@ void Lane::destroy()
     (void)hipFree(ws.wk);
     (void)hipFree(ws.wc);
     (void)hipFree(ws.ctl);
+    (void)hipFree(tile_node);
+    tile_node = nullptr;
     (void)hipFree(tot_d);
     if (tot_h) (void)hipHostFree(tot_h);
     for (int i = 0; i < 2; i++) {
@@ -103,7 +106,7 @@ Status ctx_open(int device, uint64_t budget, Ctx *&out)
     c->budget = budget ? budget : (16ull << 30);
     c->num_cus = (uint32_t)prop.multiProcessorCount;
     c->max_tiles = 1ull << 22;
-    Status st = c->lane.init(c->max_tiles);
+    Status st = c->lane.init(c->max_tiles, true);
     if (!st.good()) {
         c->lane.destroy();
         return st;
@@ -554,6 +557,10 @@ struct Runner {
             if (max_cand && cand_total + (e - c) > max_cand)
                 return Status::err(GFSM_ERR_CAP, "candidate cap reached");
             cand_total += e - c;
+            if (ctx.lane.tile_node) {  // the tiles' frontier nodes, searched by the whole GPU ahead of the step
+                HIP_TRY(launch_tile_index(io.in_off, S_prev, c, e, tc, io.ntiles, ctx.lane.tile_node, ctx.lane.stream));
+                io.tile_node = ctx.lane.tile_node;
+            }
             HIP_TRY(ctx.lane.before_launch());
             kbegin();
             HIP_TRY(launch_expand(a, grid_for(io.ntiles), ctx.lane.stream));
@@ -571,6 +578,7 @@ struct Runner {
             st = sync_totals();
             if (!st.good()) return st;
             const uint64_t keep = ctx.lane.tot_h->keep;
+            if (keep > e - c) return Status::err(GFSM_ERR_DEVICE, "internal: more nodes kept than candidates");
             const uint64_t limit = (kind == EXPAND_MID) ? cap[d] : rows_cap;
             if (keep > limit) {
                 // The level did not fit: discard this launch and halve the range.

@@ -34,7 +34,8 @@ struct Lane {
     hipEvent_t ev[2] = {nullptr, nullptr};
     hipEvent_t kev[2] = {nullptr, nullptr};
     uint64_t launches = 0;  // chained-scan launches issued on this lane
-    Status init(uint64_t max_tiles);
+    uint64_t *tile_node = nullptr;  // [2*max_tiles] per-tile frontier nodes (single-query lane only)
+    Status init(uint64_t max_tiles, bool tile_index = false);
     // Call before every chained-scan launch: clears the tile words when the
     // 16-bit launch epoch is about to repeat.
     hipError_t before_launch();

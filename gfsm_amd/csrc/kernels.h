@@ -84,6 +84,8 @@ struct StepIO {
     uint32_t digest;
     uint32_t need_nxt;         // stage the next generator per node (MID, generator not the new vertex)
     uint32_t pad;
+    const uint64_t *tile_node; // [2*ntiles] first and last frontier node of each tile (k_tile_index),
+                               // or null: each tile searches in_off itself
 };
 
 // Root step: the candidates of depth 0 (a slice of the relation index).
@@ -132,6 +134,12 @@ constexpr size_t kMaxDynLds = 150 * 1024;
 
 hipError_t launch_root(const RootArgs &a, uint32_t grid, hipStream_t s);
 hipError_t launch_expand(const ExpandArgs &a, uint32_t grid, hipStream_t s);
+// Frontier nodes of every tile of an expand step, searched ahead of it with
+// the whole GPU: out[2t] = the node holding candidate g_begin + t * tc,
+// out[2t+1] = the node holding the tile's last candidate.  A tile then reads
+// two words instead of walking the offsets with dependent loads.
+hipError_t launch_tile_index(const uint64_t *off, uint64_t S, uint64_t g_begin, uint64_t g_end, uint64_t tc,
+                             uint64_t ntiles, uint64_t *out, hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // Batched matching: many queries share every step.  Each query owns a slot in

@@ -492,7 +492,7 @@ template <uint32_t KIND, uint32_t I>
 DEV void expand_tile(const DevGraphView &g, const DepthPlanDev &dp, const uint32_t *col, const uint32_t *dcol,
                      const StepIO &io, uint64_t lt, uint64_t gt, uint64_t gt0, const ScanWS &ws, uint32_t ep,
                      uint32_t pool_words, TileShared<I> &sh, unsigned char *dyn, uint64_t &acc_cnt,
-                     uint64_t &acc_h0, uint64_t &acc_h1)
+                     uint64_t &acc_h0, uint64_t &acc_h1, bool tickets = true)
 {
     constexpr uint32_t T = kTile, TC = kTile * I, NW = TC / 64;
     const uint32_t d = io.d;
@@ -508,14 +508,16 @@ DEV void expand_tile(const DevGraphView &g, const DepthPlanDev &dp, const uint32
 
     // Frontier nodes covering [c0, c1): offsets are strictly increasing.
     if (tid < NW) s_bits[tid] = 0;
-    if (tid < 64) {
+    if (io.tile_node) {
+        if (tid < 2) sh.node[tid] = io.tile_node[2 * lt + tid];
+    } else if (tid < 64) {
         const uint32_t sub = tid & 31;
         const uint64_t key = (tid < 32) ? c0 : c1 - 1;
         const uint64_t i = frontier_search(io.in_off, io.S_prev, key, sub);
         if (sub == 0) sh.node[tid >> 5] = i;
     }
     __syncthreads();
-    if (KIND == EXPAND_LAST_COUNT) ticket_prefetch(ws, &sh.next);  // no look-back in this kind
+    if (KIND == EXPAND_LAST_COUNT && tickets) ticket_prefetch(ws, &sh.next);  // no look-back in this kind
     const uint64_t ilo = sh.node[0];
     const uint32_t nf = (uint32_t)(sh.node[1] - ilo + 1);
     const uint32_t nchk = dp.nchk;
@@ -743,12 +745,23 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(4))) void
     __shared__ TileShared<I> sh;
     const uint32_t ep = read_epoch(a.ws);
     uint64_t acc_cnt = 0, acc_h0 = 0, acc_h1 = 0;
-    tickets_init(a.ws, &sh.next);
-    for (;;) {
-        const uint64_t t = tickets_take(&sh.next);
-        if (t >= a.io.ntiles) break;
-        expand_tile<KIND, I>(a.g, a.dp, a.col, a.dcol, a.io, t, t, 0, a.ws, ep, a.pool_words, sh, smem, acc_cnt,
-                             acc_h0, acc_h1);
+    if (KIND == EXPAND_LAST_COUNT) {
+        // No tile waits on another, so tiles are dealt round-robin: a shared
+        // ticket counter that every tile passes through caps this kind's
+        // rate (measured: 1.7x faster without it).
+        for (uint64_t t = blockIdx.x; t < a.io.ntiles; t += gridDim.x) {
+            __syncthreads();  // the previous tile's readers of the tile LDS are done
+            expand_tile<KIND, I>(a.g, a.dp, a.col, a.dcol, a.io, t, t, 0, a.ws, ep, a.pool_words, sh, smem, acc_cnt,
+                                 acc_h0, acc_h1, false);
+        }
+    } else {
+        tickets_init(a.ws, &sh.next);
+        for (;;) {
+            const uint64_t t = tickets_take(&sh.next);
+            if (t >= a.io.ntiles) break;
+            expand_tile<KIND, I>(a.g, a.dp, a.col, a.dcol, a.io, t, t, 0, a.ws, ep, a.pool_words, sh, smem, acc_cnt,
+                                 acc_h0, acc_h1);
+        }
     }
     if (KIND != EXPAND_MID) flush_sums(acc_cnt, acc_h0, acc_h1, a.io.tot, sh.red);
     finish_launch(a.ws);
@@ -1115,6 +1128,21 @@ __global__ __launch_bounds__(kTile) void k_scan_test(const uint32_t *vals, uint6
     finish_launch(ws);
 }
 
+// One half-wave per search key: key 2t is tile t's first candidate, key 2t+1
+// its last.
+__global__ __launch_bounds__(kTile) void k_tile_index(const uint64_t *off, uint64_t S, uint64_t g_begin,
+                                                      uint64_t g_end, uint64_t tc, uint64_t ntiles, uint64_t *out)
+{
+    const uint64_t key_id = ((uint64_t)blockIdx.x * kTile + threadIdx.x) >> 5;
+    if (key_id >= 2 * ntiles) return;  // whole half-waves leave together (32 | kTile)
+    const uint64_t t = key_id >> 1;
+    const uint64_t c0 = g_begin + t * tc;
+    const uint64_t c1 = (c0 + tc < g_end) ? c0 + tc : g_end;
+    const uint64_t key = (key_id & 1) ? c1 - 1 : c0;
+    const uint64_t i = frontier_search(off, S, key, threadIdx.x & 31);
+    if ((threadIdx.x & 31) == 0) out[key_id] = i;
+}
+
 __global__ void k_seg_fill(uint4 *slots, uint32_t mask)
 {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= mask;
@@ -1215,6 +1243,16 @@ hipError_t launch_expand(const ExpandArgs &a, uint32_t grid, hipStream_t s)
     case EXPAND_LAST_COUNT: return launch_expand_k<EXPAND_LAST_COUNT>(a, grid, s);
     default: return launch_expand_k<EXPAND_LAST_ROWS>(a, grid, s);
     }
+}
+
+hipError_t launch_tile_index(const uint64_t *off, uint64_t S, uint64_t g_begin, uint64_t g_end, uint64_t tc,
+                             uint64_t ntiles, uint64_t *out, hipStream_t s)
+{
+    if (ntiles == 0 || S == 0 || g_end <= g_begin) return hipErrorInvalidValue;
+    const uint64_t keys_per_wg = kTile / 32;
+    const uint64_t grid = (2 * ntiles + keys_per_wg - 1) / keys_per_wg;
+    hipLaunchKernelGGL(k_tile_index, dim3((uint32_t)grid), dim3(kTile), 0, s, off, S, g_begin, g_end, tc, ntiles, out);
+    return hipGetLastError();
 }
 
 hipError_t launch_root(const RootArgs &a, uint32_t grid, hipStream_t s)

@@ -611,129 +611,139 @@ DEV void expand_tile(const DevGraphView &g, const DepthPlanDev &dp, const uint32
         cnt[k] = 0;
         gb[k] = 0;
     }
-    // Phase 2: injectivity against the node's placed vertices.
-    {
-        const uint32_t *rows = staged ? s_rows : in_rows;
+    // Phases 2-4 and the write-out read the placed vertices either from the
+    // LDS copy or, when the tile's nodes did not fit, from the level itself.
+    // Two inlined copies keep every access in its own address space: a
+    // pointer that could be either would turn the LDS reads into flat loads,
+    // which take the vector-memory path.
+    auto tail = [&](const uint32_t *__restrict__ rows) {
+        // Phase 2: injectivity against the node's placed vertices, one placed
+        // column at a time for all items, so I reads are in flight together.
+        for (uint32_t kk = 0; kk < d; kk++) {
 #pragma unroll
-        for (uint32_t k = 0; k < I; k++) {
-            const uint32_t *av = rows + jj[k] * d;
-            for (uint32_t kk = 0; kk < d; kk++) keep[k] &= (av[kk] != w[k]);
+            for (uint32_t k = 0; k < I; k++) keep[k] &= (rows[jj[k] * d + kk] != w[k]);
         }
-    }
-    // Phase 3: membership in every checked segment, I searches in lockstep.
-    for (uint32_t r = 0; r < nchk; r++) {
-        const uint32_t dist = dp.chk_dist[r];
-        uint2 sg[I];
-        if (dist && stage_meta) {
+        // Phase 3: membership in every checked segment, I searches in lockstep.
+        for (uint32_t r = 0; r < nchk; r++) {
+            const uint32_t dist = dp.chk_dist[r];
+            uint2 sg[I];
+            if (dist && stage_meta) {
 #pragma unroll
-            for (uint32_t k = 0; k < I; k++) sg[k] = s_chk[r * nf + jj[k]];
-        } else {
-            uint32_t x[I];
-            const uint32_t *rows = staged ? s_rows : in_rows;
-#pragma unroll
-            for (uint32_t k = 0; k < I; k++) x[k] = dist ? rows[jj[k] * d + (d - dist)] : w[k];
-            seg_lookup_n<I>(g, x, dp.chk_rel[r], keep, sg);
-        }
-        seg_contains_n<I>(g.nbr, sg, w, keep);
-    }
-
-    if (KIND == EXPAND_MID) {
-        // Phase 4: the survivor's next generator segment; no children, no node.
-        uint2 ns[I];
-        if (need_nxt && stage_meta) {
-#pragma unroll
-            for (uint32_t k = 0; k < I; k++) ns[k] = s_nxt[jj[k]];
-        } else {
-            uint32_t x[I];
-            const uint32_t *rows = staged ? s_rows : in_rows;
-#pragma unroll
-            for (uint32_t k = 0; k < I; k++) x[k] = need_nxt ? rows[jj[k] * d + (d + 1 - dp.nxt_dist)] : w[k];
-            seg_lookup_n<I>(g, x, dp.nxt_rel, keep, ns);
-        }
-#pragma unroll
-        for (uint32_t k = 0; k < I; k++) {
-            if (!keep[k]) continue;
-            cnt[k] = ns[k].y - ns[k].x;
-            gb[k] = ns[k].x;
-            keep[k] = cnt[k] > 0;
-        }
-    }
-
-    if (KIND != EXPAND_LAST_COUNT) {
-        // Compaction: wave-level ballot/mbcnt and DPP scans, then the
-        // tile's prefix from the look-back, then each item's slot.
-        ScanSmem<I> &sm = sh.sm;
-        uint32_t kpre[I], cpre[I];
-#pragma unroll
-        for (uint32_t k = 0; k < I; k++) {
-            const uint64_t m = __ballot(keep[k]);
-            kpre[k] = mask_prefix(m);
-            const uint32_t cinc = wave_incl_scan(cnt[k]);
-            cpre[k] = cinc - cnt[k];
-            if (lane == 63) {
-                sm.wk[k][wave] = (uint32_t)__popcll(m);
-                sm.wc[k][wave] = cinc;
-            }
-        }
-        __syncthreads();
-        tile_prefix<I>(gt, gt0, ws, ep, sm, &sh.next);
-        uint64_t bk = sm.ex[0], bc = sm.ex[1];
-        const uint32_t *rows = staged ? s_rows : in_rows;
-#pragma unroll
-        for (uint32_t k = 0; k < I; k++) {
-            uint64_t ok_ = bk, oc_ = bc;
-#pragma unroll
-            for (uint32_t ww = 0; ww < kWaves; ww++) {
-                if (ww < wave) {
-                    ok_ += sm.wk[k][ww];
-                    oc_ += sm.wc[k][ww];
-                }
-                bk += sm.wk[k][ww];
-                bc += sm.wc[k][ww];
-            }
-            if (!keep[k]) continue;
-            const uint64_t idx = ok_ + kpre[k];
-            if (idx >= io.cap) continue;
-            const uint32_t *av = rows + jj[k] * d;
-            if (KIND == EXPAND_MID) {
-                uint32_t *ov = io.out_vec + idx * (d + 1);
-                for (uint32_t kk = 0; kk < d; kk++) ov[kk] = av[kk];
-                ov[d] = w[k];
-                io.out_off[idx] = oc_ + cpre[k];
-                io.out_gbeg[idx] = gb[k];
+                for (uint32_t k = 0; k < I; k++) sg[k] = s_chk[r * nf + jj[k]];
             } else {
-                uint32_t *row = io.rows + idx * io.q;
-                for (uint32_t kk = 0; kk < d; kk++) row[col[kk]] = av[kk];
-                row[col[d]] = w[k];
+                uint32_t x[I];
+#pragma unroll
+                for (uint32_t k = 0; k < I; k++) x[k] = dist ? rows[jj[k] * d + (d - dist)] : w[k];
+                seg_lookup_n<I>(g, x, dp.chk_rel[r], keep, sg);
+            }
+            seg_contains_n<I>(g.nbr, sg, w, keep);
+        }
+
+        if (KIND == EXPAND_MID) {
+            // Phase 4: the survivor's next generator segment; no children, no node.
+            uint2 ns[I];
+            if (need_nxt && stage_meta) {
+#pragma unroll
+                for (uint32_t k = 0; k < I; k++) ns[k] = s_nxt[jj[k]];
+            } else {
+                uint32_t x[I];
+#pragma unroll
+                for (uint32_t k = 0; k < I; k++) x[k] = need_nxt ? rows[jj[k] * d + (d + 1 - dp.nxt_dist)] : w[k];
+                seg_lookup_n<I>(g, x, dp.nxt_rel, keep, ns);
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < I; k++) {
+                if (!keep[k]) continue;
+                cnt[k] = ns[k].y - ns[k].x;
+                gb[k] = ns[k].x;
+                keep[k] = cnt[k] > 0;
             }
         }
-        if (lt == io.ntiles - 1 && tid == 0) {
-            io.tot->keep = sm.tot[0];
-            io.tot->cnt = sm.tot[1];
-            if (KIND == EXPAND_MID && sm.tot[0] <= io.cap) io.out_off[sm.tot[0]] = sm.tot[1];
-        }
-    } else {
+
+        if (KIND != EXPAND_LAST_COUNT) {
+            // Compaction: wave-level ballot/mbcnt and DPP scans, then the
+            // tile's prefix from the look-back, then each item's slot.
+            ScanSmem<I> &sm = sh.sm;
+            uint32_t kpre[I], cpre[I];
 #pragma unroll
-        for (uint32_t k = 0; k < I; k++) acc_cnt += keep[k] ? 1u : 0u;
-    }
-    if (KIND != EXPAND_MID && io.digest) {
-        const uint32_t *rows = staged ? s_rows : in_rows;
-#pragma unroll
-        for (uint32_t k = 0; k < I; k++) {
-            if (!keep[k]) continue;
-            uint64_t x0 = 0x9E3779B97F4A7C15ull, x1 = 0xD1B54A32D192ED03ull;
-            for (uint32_t c = 0; c < io.q; c++) {
-                const uint32_t dc = dcol[c];
-                const uint32_t val = (dc == d) ? w[k] : rows[jj[k] * d + dc];
-                const uint64_t xv = (uint64_t)val | ((uint64_t)c << 32);
-                x0 = mix64(x0 ^ xv);
-                x1 = mix64(x1 ^ xv);
+            for (uint32_t k = 0; k < I; k++) {
+                const uint64_t m = __ballot(keep[k]);
+                kpre[k] = mask_prefix(m);
+                const uint32_t cinc = wave_incl_scan(cnt[k]);
+                cpre[k] = cinc - cnt[k];
+                if (lane == 63) {
+                    sm.wk[k][wave] = (uint32_t)__popcll(m);
+                    sm.wc[k][wave] = cinc;
+                }
             }
-            acc_h0 += x0;
-            acc_h1 += x1;
+            __syncthreads();
+            tile_prefix<I>(gt, gt0, ws, ep, sm, &sh.next);
+            uint64_t bk = sm.ex[0], bc = sm.ex[1];
+            uint64_t idx[I];
+            bool put[I];
+#pragma unroll
+            for (uint32_t k = 0; k < I; k++) {
+                uint64_t ok_ = bk, oc_ = bc;
+#pragma unroll
+                for (uint32_t ww = 0; ww < kWaves; ww++) {
+                    if (ww < wave) {
+                        ok_ += sm.wk[k][ww];
+                        oc_ += sm.wc[k][ww];
+                    }
+                    bk += sm.wk[k][ww];
+                    bc += sm.wc[k][ww];
+                }
+                idx[k] = ok_ + kpre[k];
+                put[k] = keep[k] && idx[k] < io.cap;
+                if (KIND == EXPAND_MID && put[k]) {
+                    io.out_vec[idx[k] * (d + 1) + d] = w[k];
+                    io.out_off[idx[k]] = oc_ + cpre[k];
+                    io.out_gbeg[idx[k]] = gb[k];
+                } else if (KIND == EXPAND_LAST_ROWS && put[k]) {
+                    io.rows[idx[k] * io.q + col[d]] = w[k];
+                }
+            }
+            // The placed vertices, one column at a time for all items.
+            for (uint32_t kk = 0; kk < d; kk++) {
+                const uint32_t ck = (KIND == EXPAND_MID) ? kk : col[kk];
+                const uint64_t wd = (KIND == EXPAND_MID) ? d + 1 : io.q;
+                uint32_t *out = (KIND == EXPAND_MID) ? io.out_vec : io.rows;
+#pragma unroll
+                for (uint32_t k = 0; k < I; k++)
+                    if (put[k]) out[idx[k] * wd + ck] = rows[jj[k] * d + kk];
+            }
+            if (lt == io.ntiles - 1 && tid == 0) {
+                io.tot->keep = sm.tot[0];
+                io.tot->cnt = sm.tot[1];
+                if (KIND == EXPAND_MID && sm.tot[0] <= io.cap) io.out_off[sm.tot[0]] = sm.tot[1];
+            }
+        } else {
+#pragma unroll
+            for (uint32_t k = 0; k < I; k++) acc_cnt += keep[k] ? 1u : 0u;
         }
-    }
+        if (KIND != EXPAND_MID && io.digest) {
+#pragma unroll
+            for (uint32_t k = 0; k < I; k++) {
+                if (!keep[k]) continue;
+                uint64_t x0 = 0x9E3779B97F4A7C15ull, x1 = 0xD1B54A32D192ED03ull;
+                for (uint32_t c = 0; c < io.q; c++) {
+                    const uint32_t dc = dcol[c];
+                    const uint32_t val = (dc == d) ? w[k] : rows[jj[k] * d + dc];
+                    const uint64_t xv = (uint64_t)val | ((uint64_t)c << 32);
+                    x0 = mix64(x0 ^ xv);
+                    x1 = mix64(x1 ^ xv);
+                }
+                acc_h0 += x0;
+                acc_h1 += x1;
+            }
+        }
+    };
+    if (staged)
+        tail(s_rows);
+    else
+        tail(in_rows);
 }
+
 
 // Four waves per SIMD (at most 128 VGPRs): the kernel is bound by the latency
 // of dependent loads, and the extra resident workgroup hides more of it than

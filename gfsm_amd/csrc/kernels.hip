@@ -681,6 +681,13 @@ DEV void expand_tile(const DevGraphView &g, const DepthPlanDev &dp, const uint32
             uint64_t bk = sm.ex[0], bc = sm.ex[1];
             uint64_t idx[I];
             bool put[I];
+            // MID rows of one wave and item are consecutive in the next
+            // level: staged in this wave's share of s_seg (free after phase
+            // 1) and written out as whole words by all lanes, instead of
+            // each lane storing its row with a stride of d + 1 words.
+            const uint32_t row_w = d + 1;
+            const bool coalesce = KIND == EXPAND_MID && row_w * 64 <= TC / kWaves;
+            uint32_t *wst = s_seg + wave * (TC / kWaves);
 #pragma unroll
             for (uint32_t k = 0; k < I; k++) {
                 uint64_t ok_ = bk, oc_ = bc;
@@ -696,21 +703,48 @@ DEV void expand_tile(const DevGraphView &g, const DepthPlanDev &dp, const uint32
                 idx[k] = ok_ + kpre[k];
                 put[k] = keep[k] && idx[k] < io.cap;
                 if (KIND == EXPAND_MID && put[k]) {
-                    io.out_vec[idx[k] * (d + 1) + d] = w[k];
                     io.out_off[idx[k]] = oc_ + cpre[k];
                     io.out_gbeg[idx[k]] = gb[k];
                 } else if (KIND == EXPAND_LAST_ROWS && put[k]) {
                     io.rows[idx[k] * io.q + col[d]] = w[k];
                 }
-            }
-            // The placed vertices, one column at a time for all items.
-            for (uint32_t kk = 0; kk < d; kk++) {
-                const uint32_t ck = (KIND == EXPAND_MID) ? kk : col[kk];
-                const uint64_t wd = (KIND == EXPAND_MID) ? d + 1 : io.q;
-                uint32_t *out = (KIND == EXPAND_MID) ? io.out_vec : io.rows;
+                if (KIND == EXPAND_MID && coalesce) {
+                    if (put[k]) {
+                        uint32_t *sr = wst + kpre[k] * row_w;
+                        for (uint32_t kk = 0; kk < d; kk++) sr[kk] = rows[jj[k] * d + kk];
+                        sr[d] = w[k];
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    // rows [ok_, ok_ + nrow) of the level, capped at its capacity
+                    uint64_t nrow = (uint64_t)__popcll(__ballot(keep[k]));
+                    nrow = ok_ >= io.cap ? 0 : (nrow < io.cap - ok_ ? nrow : io.cap - ok_);
+                    const uint32_t words = (uint32_t)nrow * row_w;  // at most 64 * I
+                    uint32_t *dst = io.out_vec + ok_ * row_w;
+                    uint32_t v[I];
 #pragma unroll
-                for (uint32_t k = 0; k < I; k++)
-                    if (put[k]) out[idx[k] * wd + ck] = rows[jj[k] * d + kk];
+                    for (uint32_t i = 0; i < I; i++) v[i] = (lane + 64 * i < words) ? wst[lane + 64 * i] : 0u;
+#pragma unroll
+                    for (uint32_t i = 0; i < I; i++)
+                        if (lane + 64 * i < words) dst[lane + 64 * i] = v[i];
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                } else if (KIND == EXPAND_MID && put[k]) {
+                    io.out_vec[idx[k] * row_w + d] = w[k];
+                }
+            }
+            // Otherwise the placed vertices, one column at a time for all items.
+            if (!(KIND == EXPAND_MID && coalesce)) {
+                for (uint32_t kk = 0; kk < d; kk++) {
+                    const uint32_t ck = (KIND == EXPAND_MID) ? kk : col[kk];
+                    const uint64_t wd = (KIND == EXPAND_MID) ? d + 1 : io.q;
+                    uint32_t *out = (KIND == EXPAND_MID) ? io.out_vec : io.rows;
+#pragma unroll
+                    for (uint32_t k = 0; k < I; k++)
+                        if (put[k]) out[idx[k] * wd + ck] = rows[jj[k] * d + kk];
+                }
             }
             if (lt == io.ntiles - 1 && tid == 0) {
                 io.tot->keep = sm.tot[0];

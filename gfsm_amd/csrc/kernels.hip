@@ -32,6 +32,17 @@ namespace gfsm {
 
 #define DEV __device__ __forceinline__
 
+// Pointers into the global address space.  Buffer pointers that a kernel
+// loads from memory (the batch slots) are generic, and accesses through
+// generic pointers become flat instructions; these keep them global.
+template <class T>
+using gptr_t = __attribute__((address_space(1))) T *;
+template <class T>
+DEV gptr_t<T> gp(T *p)
+{
+    return (gptr_t<T>)p;
+}
+
 constexpr uint32_t kWaves = kTile / 64;
 constexpr uint32_t kNodeLk = 4;  // node-setup lookups a thread keeps in flight
 
@@ -359,7 +370,7 @@ DEV void finish_launch(const ScanWS &ws)
 
 // Sum (count, digest0, digest1) over the workgroup and add it to `tot`: one
 // set of atomics per workgroup, not per wave.
-DEV void flush_sums(uint64_t cnt, uint64_t h0, uint64_t h1, Totals *tot, uint64_t (*s_red)[kWaves])
+DEV void flush_sums(uint64_t cnt, uint64_t h0, uint64_t h1, Totals *tot_ptr, uint64_t (*s_red)[kWaves])
 {
     const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
     cnt = wave_sum64(cnt);
@@ -378,9 +389,10 @@ DEV void flush_sums(uint64_t cnt, uint64_t h0, uint64_t h1, Totals *tot, uint64_
             a += s_red[1][w];
             b += s_red[2][w];
         }
-        if (c) atomicAdd((unsigned long long *)&tot->count, (unsigned long long)c);
-        if (a) atomicAdd((unsigned long long *)&tot->dg0, (unsigned long long)a);
-        if (b) atomicAdd((unsigned long long *)&tot->dg1, (unsigned long long)b);
+        const auto tot = gp(tot_ptr);
+        if (c) __hip_atomic_fetch_add(&tot->count, c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (a) __hip_atomic_fetch_add(&tot->dg0, a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (b) __hip_atomic_fetch_add(&tot->dg1, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -401,8 +413,8 @@ struct TileShared {
 
 // Root tile `lt` of a root step (global tile gt, the step's scan starting at
 // gt0): keep candidates that carry every self-loop relation of the root.
-template <uint32_t I>
-DEV void root_tile(const DevGraphView &g, const RootIO &io, uint32_t nchk, const uint32_t *chk_rel, uint64_t lt,
+template <uint32_t I, class CP>
+DEV void root_tile(const DevGraphView &g, const RootIO &io, uint32_t nchk, CP chk_rel, uint64_t lt,
                    uint64_t gt, uint64_t gt0, const ScanWS &ws, uint32_t ep, ScanSmem<I> &sm, uint32_t *s_next)
 {
     bool keep[I];
@@ -414,8 +426,8 @@ DEV void root_tile(const DevGraphView &g, const RootIO &io, uint32_t nchk, const
         v[k] = 0;
         cnt[k] = 0;
         if (c < io.n) {
-            v[k] = io.cand_vtx[c];
-            cnt[k] = io.cand_cnt[c];
+            v[k] = gp(io.cand_vtx)[c];
+            cnt[k] = gp(io.cand_cnt)[c];
             keep[k] = cnt[k] > 0;
         }
     }
@@ -433,15 +445,15 @@ DEV void root_tile(const DevGraphView &g, const RootIO &io, uint32_t nchk, const
     for (uint32_t k = 0; k < I; k++) {
         if (keep[k] && kk[k] < io.cap) {
             const uint64_t c = lt * (kTile * I) + k * kTile + threadIdx.x;
-            io.out_vec[kk[k]] = v[k];
-            io.out_off[kk[k]] = off[k];
-            io.out_gbeg[kk[k]] = io.cand_beg[c];
+            gp(io.out_vec)[kk[k]] = v[k];
+            gp(io.out_off)[kk[k]] = off[k];
+            gp(io.out_gbeg)[kk[k]] = gp(io.cand_beg)[c];
         }
     }
     if (lt == io.ntiles - 1 && threadIdx.x == 0) {
-        io.tot->keep = rk;
-        io.tot->cnt = rc;
-        if (rk <= io.cap) io.out_off[rk] = rc;
+        gp(io.tot)->keep = rk;
+        gp(io.tot)->cnt = rc;
+        if (rk <= io.cap) gp(io.out_off)[rk] = rc;
     }
 }
 
@@ -465,7 +477,8 @@ __global__ __launch_bounds__(kTile) void k_root(RootArgs a)
 
 // Largest i in [0, S) with off[i] <= key, found by half a wave (32 lanes,
 // `sub` = lane within the half) in 32-ary steps; off[0] = 0 <= key.
-DEV uint64_t frontier_search(const uint64_t *__restrict__ off, uint64_t S, uint64_t key, uint32_t sub)
+template <class P>
+DEV uint64_t frontier_search(P off, uint64_t S, uint64_t key, uint32_t sub)
 {
     uint64_t lo = 0, hi = S;
     while (hi - lo > 1) {
@@ -488,8 +501,8 @@ DEV uint64_t frontier_search(const uint64_t *__restrict__ off, uint64_t S, uint6
 // tile, the checked ancestor segments, the next generator segment and the
 // placed vertices, as far as they fit.  LAST_COUNT adds its embeddings and
 // digest to the thread's accumulators; the other kinds write their output.
-template <uint32_t KIND, uint32_t I>
-DEV void expand_tile(const DevGraphView &g, const DepthPlanDev &dp, const uint32_t *col, const uint32_t *dcol,
+template <uint32_t KIND, uint32_t I, class DP, class CP>
+DEV void expand_tile(const DevGraphView &g, const DP &dp, CP col, CP dcol,
                      const StepIO &io, uint64_t lt, uint64_t gt, uint64_t gt0, const ScanWS &ws, uint32_t ep,
                      uint32_t pool_words, TileShared<I> &sh, unsigned char *dyn, uint64_t &acc_cnt,
                      uint64_t &acc_h0, uint64_t &acc_h1, bool tickets = true)
@@ -505,15 +518,22 @@ DEV void expand_tile(const DevGraphView &g, const DepthPlanDev &dp, const uint32
     const uint64_t c0 = io.g_begin + lt * TC;
     const uint64_t c1 = (c0 + TC < io.g_end) ? c0 + TC : io.g_end;
     const uint32_t n = (uint32_t)(c1 - c0);
+    const auto in_off = gp(io.in_off);
+    const auto in_gbeg = gp(io.in_gbeg);
+    const auto out_vec = gp(io.out_vec);
+    const auto out_off = gp(io.out_off);
+    const auto out_gbeg = gp(io.out_gbeg);
+    const auto out_rows = gp(io.rows);
+    const auto tot = gp(io.tot);
 
     // Frontier nodes covering [c0, c1): offsets are strictly increasing.
     if (tid < NW) s_bits[tid] = 0;
     if (io.tile_node) {
-        if (tid < 2) sh.node[tid] = io.tile_node[2 * lt + tid];
+        if (tid < 2) sh.node[tid] = gp(io.tile_node)[2 * lt + tid];
     } else if (tid < 64) {
         const uint32_t sub = tid & 31;
         const uint64_t key = (tid < 32) ? c0 : c1 - 1;
-        const uint64_t i = frontier_search(io.in_off, io.S_prev, key, sub);
+        const uint64_t i = frontier_search(in_off, io.S_prev, key, sub);
         if (sub == 0) sh.node[tid >> 5] = i;
     }
     __syncthreads();
@@ -528,17 +548,17 @@ DEV void expand_tile(const DevGraphView &g, const DepthPlanDev &dp, const uint32
     uint2 *s_chk = reinterpret_cast<uint2 *>(s_pool);  // [nchk][nf]
     uint2 *s_nxt = s_chk + nchk * nf;                   // [nf]
     uint32_t *s_rows = s_pool + meta_words;             // [nf][d]
-    const uint32_t *in_rows = io.in_vec + ilo * d;      // the tile's node rows, contiguous
+    const auto in_rows = gp(io.in_vec) + ilo * d;       // the tile's node rows, contiguous
 
     if (staged)
         for (uint32_t u = tid; u < nf * d; u += T) s_rows[u] = in_rows[u];
     // Per frontier node: start bit, generator base.
     for (uint32_t j = tid; j < nf; j += T) {
         const uint64_t i = ilo + j;
-        const uint64_t o = io.in_off[i];
+        const uint64_t o = in_off[i];
         const uint32_t first = (o > c0) ? (uint32_t)(o - c0) : 0u;
         atomicOr((unsigned long long *)&s_bits[first >> 6], 1ull << (first & 63));
-        s_seg[j] = io.in_gbeg[i] + (uint32_t)(c0 - o);
+        s_seg[j] = in_gbeg[i] + (uint32_t)(c0 - o);
     }
     // Staged segments: every (node, lookup) pair of the tile -- the node's
     // checked ancestors and its next generator -- striped over the threads
@@ -616,7 +636,7 @@ DEV void expand_tile(const DevGraphView &g, const DepthPlanDev &dp, const uint32
     // Two inlined copies keep every access in its own address space: a
     // pointer that could be either would turn the LDS reads into flat loads,
     // which take the vector-memory path.
-    auto tail = [&](const uint32_t *__restrict__ rows) {
+    auto tail = [&](auto rows) {
         // Phase 2: injectivity against the node's placed vertices, one placed
         // column at a time for all items, so I reads are in flight together.
         for (uint32_t kk = 0; kk < d; kk++) {
@@ -703,10 +723,10 @@ DEV void expand_tile(const DevGraphView &g, const DepthPlanDev &dp, const uint32
                 idx[k] = ok_ + kpre[k];
                 put[k] = keep[k] && idx[k] < io.cap;
                 if (KIND == EXPAND_MID && put[k]) {
-                    io.out_off[idx[k]] = oc_ + cpre[k];
-                    io.out_gbeg[idx[k]] = gb[k];
+                    out_off[idx[k]] = oc_ + cpre[k];
+                    out_gbeg[idx[k]] = gb[k];
                 } else if (KIND == EXPAND_LAST_ROWS && put[k]) {
-                    io.rows[idx[k] * io.q + col[d]] = w[k];
+                    out_rows[idx[k] * io.q + col[d]] = w[k];
                 }
                 if (KIND == EXPAND_MID && coalesce) {
                     if (put[k]) {
@@ -721,7 +741,7 @@ DEV void expand_tile(const DevGraphView &g, const DepthPlanDev &dp, const uint32
                     uint64_t nrow = (uint64_t)__popcll(__ballot(keep[k]));
                     nrow = ok_ >= io.cap ? 0 : (nrow < io.cap - ok_ ? nrow : io.cap - ok_);
                     const uint32_t words = (uint32_t)nrow * row_w;  // at most 64 * I
-                    uint32_t *dst = io.out_vec + ok_ * row_w;
+                    const auto dst = out_vec + ok_ * row_w;
                     uint32_t v[I];
 #pragma unroll
                     for (uint32_t i = 0; i < I; i++) v[i] = (lane + 64 * i < words) ? wst[lane + 64 * i] : 0u;
@@ -732,7 +752,7 @@ DEV void expand_tile(const DevGraphView &g, const DepthPlanDev &dp, const uint32
                     __builtin_amdgcn_wave_barrier();
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 } else if (KIND == EXPAND_MID && put[k]) {
-                    io.out_vec[idx[k] * row_w + d] = w[k];
+                    out_vec[idx[k] * row_w + d] = w[k];
                 }
             }
             // Otherwise the placed vertices, one column at a time for all items.
@@ -740,16 +760,16 @@ DEV void expand_tile(const DevGraphView &g, const DepthPlanDev &dp, const uint32
                 for (uint32_t kk = 0; kk < d; kk++) {
                     const uint32_t ck = (KIND == EXPAND_MID) ? kk : col[kk];
                     const uint64_t wd = (KIND == EXPAND_MID) ? d + 1 : io.q;
-                    uint32_t *out = (KIND == EXPAND_MID) ? io.out_vec : io.rows;
+                    const auto out = (KIND == EXPAND_MID) ? out_vec : out_rows;
 #pragma unroll
                     for (uint32_t k = 0; k < I; k++)
                         if (put[k]) out[idx[k] * wd + ck] = rows[jj[k] * d + kk];
                 }
             }
             if (lt == io.ntiles - 1 && tid == 0) {
-                io.tot->keep = sm.tot[0];
-                io.tot->cnt = sm.tot[1];
-                if (KIND == EXPAND_MID && sm.tot[0] <= io.cap) io.out_off[sm.tot[0]] = sm.tot[1];
+                tot->keep = sm.tot[0];
+                tot->cnt = sm.tot[1];
+                if (KIND == EXPAND_MID && sm.tot[0] <= io.cap) out_off[sm.tot[0]] = sm.tot[1];
             }
         } else {
 #pragma unroll
@@ -1117,7 +1137,7 @@ __global__ __launch_bounds__(kTile) void k_batch_step(BatchCfg cfg)
         const BatchSlot &B = cfg.slots[s];
         const uint32_t step = B.step;
         const uint64_t t0 = B.tile0;
-        const QueryPlanDev *plan = B.plan;
+        const auto plan = gp(B.plan);  // loaded from memory: keep its reads global
         if (step == STEP_ROOT) {
             root_tile<I>(cfg.g, B.rio, plan->nroot_chk, plan->root_chk_rel, t - t0, t, t0, cfg.ws, ep, sh.sm, &sh.next);
         } else if (step == EXPAND_MID) {

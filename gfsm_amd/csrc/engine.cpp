@@ -118,6 +118,7 @@ Status ctx_open(int device, uint64_t budget, Ctx *&out)
 static void free_mirror(DevGraph *dg)
 {
     (void)hipSetDevice(dg->device);
+    for (auto &kv : dg->dense) (void)hipFree(kv.second);
     (void)hipFree(dg->nbr);
     (void)hipFree(dg->dir_ptr);
     (void)hipFree(dg->dir);
@@ -305,6 +306,9 @@ namespace {
 // batch engine (whose steps cost nothing per query).
 constexpr uint64_t kCapStartDiv = 8;
 
+// Device bytes of dense segment tables one resident graph may hold.
+constexpr uint64_t kDenseMaxBytes = 4ull << 30;
+
 // One plain match in flight: level buffers carved from the arena, the lane,
 // and the depth-first chunk recursion.  Level k (nodes with k+1 placed
 // vertices) holds rows of width k+1, candidate offsets and generator begins.
@@ -355,6 +359,37 @@ struct Runner {
         const uint64_t by_ws = ctx.max_tiles * kTile;  // enough tiles at any tile size
         const uint64_t by_val = kValueMask / std::max<uint64_t>(1, data.max_degree);
         return std::max<uint64_t>(kTile * 8, std::min(by_ws, by_val));
+    }
+
+    // Dense segment table of relation `rel` for a launch testing `ncand`
+    // candidates: cached on the mirror; built when the launch makes at least
+    // V/8 lookups, within kDenseMaxBytes per mirror.  Null: hash lookups.
+    const uint2 *dense_table(uint32_t rel, uint64_t ncand)
+    {
+        DevGraph &m = const_cast<DevGraph &>(dg);
+        auto it = m.dense.find(rel);
+        if (it != m.dense.end()) return it->second;
+        const uint64_t bytes = (uint64_t)data.V * sizeof(uint2);
+        if (ncand < data.V / 8 || m.dense_bytes + bytes > kDenseMaxBytes) return nullptr;
+        uint2 *t = nullptr;
+        if (hipMalloc(&t, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        uint64_t lo = 0, hi = 0;
+        const int64_t k = data.rel_index(rel);
+        if (k >= 0) {
+            lo = data.rel_off[k];
+            hi = data.rel_off[k + 1];
+        }
+        if (launch_dense_seg(dg.rel_vtx + lo, dg.rel_beg + lo, dg.rel_cnt + lo, hi - lo, data.V, t,
+                             ctx.lane.stream) != hipSuccess) {
+            (void)hipFree(t);
+            return nullptr;
+        }
+        m.dense[rel] = t;
+        m.dense_bytes += bytes;
+        return t;
     }
 
     uint32_t grid_for(uint64_t ntiles) const
@@ -527,6 +562,7 @@ struct Runner {
             std::memcpy(a.dcol, plan.dev.dcol, sizeof(a.dcol));
             StepIO &io = a.io;
             io.need_nxt = need_nxt(kind, d);
+            if (kind == EXPAND_MID && !io.need_nxt) io.nxt_seg = dense_table(plan.dev.depth[d].nxt_rel, e - c);
             io.d = d;
             io.q = q;
             io.kind = kind;

@@ -7,6 +7,7 @@
 #include <memory>
 #include <mutex>
 #include <set>
+#include <unordered_map>
 
 #include "host.h"
 #include "kernels.h"
@@ -21,6 +22,10 @@ struct DevGraph {
     uint4 *seg = nullptr;  // segment hash table (see DevGraphView)
     DevGraphView view{};
     size_t bytes = 0;
+    // Dense per-relation segment tables ({begin, end} for every vertex), built
+    // on first use by the single-query path and kept while the mirror lives.
+    std::unordered_map<uint32_t, uint2 *> dense;
+    uint64_t dense_bytes = 0;
 };
 
 // A stream with its own chained-scan workspace and result block.  Kernels of

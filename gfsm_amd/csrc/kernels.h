@@ -86,6 +86,8 @@ struct StepIO {
     uint32_t pad;
     const uint64_t *tile_node; // [2*ntiles] first and last frontier node of each tile (k_tile_index),
                                // or null: each tile searches in_off itself
+    const uint2 *nxt_seg;      // [V] {begin, end} of every vertex's segment of the depth's next
+                               // relation (MID, next generator = the new vertex), or null: hash lookups
 };
 
 // Root step: the candidates of depth 0 (a slice of the relation index).
@@ -134,6 +136,11 @@ constexpr size_t kMaxDynLds = 150 * 1024;
 
 hipError_t launch_root(const RootArgs &a, uint32_t grid, hipStream_t s);
 hipError_t launch_expand(const ExpandArgs &a, uint32_t grid, hipStream_t s);
+// Dense segment table of one relation: out[v] = {begin, end} of v's segment
+// of that relation ({0, 0} when v has none).  vtx/beg/cnt are the
+// relation's slice of the relation index; out has V entries.
+hipError_t launch_dense_seg(const uint32_t *vtx, const uint32_t *beg, const uint32_t *cnt, uint64_t n, uint32_t V,
+                            uint2 *out, hipStream_t s);
 // Frontier nodes of every tile of an expand step, searched ahead of it with
 // the whole GPU: out[2t] = the node holding candidate g_begin + t * tc,
 // out[2t+1] = the node holding the tile's last candidate.  A tile then reads

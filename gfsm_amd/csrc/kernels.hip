@@ -665,6 +665,14 @@ DEV void expand_tile(const DevGraphView &g, const DP &dp, CP col, CP dcol,
             if (need_nxt && stage_meta) {
 #pragma unroll
                 for (uint32_t k = 0; k < I; k++) ns[k] = s_nxt[jj[k]];
+            } else if (!need_nxt && io.nxt_seg) {
+                // the new vertex's segment: one read of an L2-sized table
+                const auto tab = gp(reinterpret_cast<const uint64_t *>(io.nxt_seg));  // {begin, end} words
+#pragma unroll
+                for (uint32_t k = 0; k < I; k++) {
+                    const uint64_t v = keep[k] ? tab[w[k]] : 0ull;
+                    ns[k] = make_uint2((uint32_t)v, (uint32_t)(v >> 32));
+                }
             } else {
                 uint32_t x[I];
 #pragma unroll
@@ -1192,6 +1200,12 @@ __global__ __launch_bounds__(kTile) void k_scan_test(const uint32_t *vals, uint6
     finish_launch(ws);
 }
 
+__global__ void k_dense_seg(const uint32_t *vtx, const uint32_t *beg, const uint32_t *cnt, uint64_t n, uint2 *out)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        out[vtx[i]] = make_uint2(beg[i], beg[i] + cnt[i]);
+}
+
 // One half-wave per search key: key 2t is tile t's first candidate, key 2t+1
 // its last.
 __global__ __launch_bounds__(kTile) void k_tile_index(const uint64_t *off, uint64_t S, uint64_t g_begin,
@@ -1307,6 +1321,16 @@ hipError_t launch_expand(const ExpandArgs &a, uint32_t grid, hipStream_t s)
     case EXPAND_LAST_COUNT: return launch_expand_k<EXPAND_LAST_COUNT>(a, grid, s);
     default: return launch_expand_k<EXPAND_LAST_ROWS>(a, grid, s);
     }
+}
+
+hipError_t launch_dense_seg(const uint32_t *vtx, const uint32_t *beg, const uint32_t *cnt, uint64_t n, uint32_t V,
+                            uint2 *out, hipStream_t s)
+{
+    hipError_t e = hipMemsetAsync(out, 0, (size_t)V * sizeof(uint2), s);
+    if (e != hipSuccess || n == 0) return e;
+    const uint64_t grid = (n + 255) / 256 < 8192 ? (n + 255) / 256 : 8192;
+    hipLaunchKernelGGL(k_dense_seg, dim3((uint32_t)grid), dim3(256), 0, s, vtx, beg, cnt, n, out);
+    return hipGetLastError();
 }
 
 hipError_t launch_tile_index(const uint64_t *off, uint64_t S, uint64_t g_begin, uint64_t g_end, uint64_t tc,

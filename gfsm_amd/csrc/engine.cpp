@@ -562,7 +562,9 @@ struct Runner {
             std::memcpy(a.dcol, plan.dev.dcol, sizeof(a.dcol));
             StepIO &io = a.io;
             io.need_nxt = need_nxt(kind, d);
-            if (kind == EXPAND_MID && !io.need_nxt) io.nxt_seg = dense_table(plan.dev.depth[d].nxt_rel, e - c);
+            if (kind == EXPAND_MID) io.nxt_seg = dense_table(plan.dev.depth[d].nxt_rel, e - c);
+            for (uint32_t r = 0; r < std::min<uint32_t>(plan.dev.depth[d].nchk, kDenseChk); r++)
+                io.chk_seg[r] = dense_table(plan.dev.depth[d].chk_rel[r], e - c);
             io.d = d;
             io.q = q;
             io.kind = kind;

@@ -571,11 +571,13 @@ DEV void expand_tile(const DevGraphView &g, const DP &dp, CP col, CP dcol,
         for (uint32_t u0 = tid; u0 < total; u0 += T * L) {
             uint32_t x[L], rel[L], dst[L];
             bool act[L];
+            const uint2 *tab[L];
 #pragma unroll
             for (uint32_t k = 0; k < L; k++) {
                 const uint32_t u = u0 + k * T;
                 act[k] = false;
                 x[k] = rel[k] = dst[k] = 0;
+                tab[k] = nullptr;
                 if (u < total) {
                     const uint32_t j = u / nl, r = u - j * nl;
                     // row column of the vertex whose segment is looked up
@@ -585,14 +587,23 @@ DEV void expand_tile(const DevGraphView &g, const DP &dp, CP col, CP dcol,
                     const uint32_t col = (r < nchk) ? d - dp.chk_dist[r] : d + 1 - dp.nxt_dist;
                     rel[k] = (r < nchk) ? dp.chk_rel[r] : dp.nxt_rel;
                     dst[k] = (r < nchk) ? r * nf + j : nchk * nf + j;  // s_chk[r][j] or s_nxt[j]
+                    tab[k] = (r < nchk) ? (r < kDenseChk ? io.chk_seg[r] : nullptr) : io.nxt_seg;
                     if (act[k]) x[k] = in_rows[(uint64_t)j * d + col];
                 }
             }
+            // dense tables where the relation has one, hash lookups for the rest
+            bool hact[L];
+            uint64_t dv[L];
+#pragma unroll
+            for (uint32_t k = 0; k < L; k++) {
+                hact[k] = act[k] && !tab[k];
+                dv[k] = (act[k] && tab[k]) ? gp(reinterpret_cast<const uint64_t *>(tab[k]))[x[k]] : 0ull;
+            }
             uint2 sg[L];
-            seg_lookup_r<L>(g, x, rel, act, sg);
+            seg_lookup_r<L>(g, x, rel, hact, sg);
 #pragma unroll
             for (uint32_t k = 0; k < L; k++)
-                if (act[k]) s_chk[dst[k]] = sg[k];
+                if (act[k]) s_chk[dst[k]] = tab[k] ? make_uint2((uint32_t)dv[k], (uint32_t)(dv[k] >> 32)) : sg[k];
         }
     }
     __syncthreads();
@@ -654,7 +665,17 @@ DEV void expand_tile(const DevGraphView &g, const DP &dp, CP col, CP dcol,
                 uint32_t x[I];
 #pragma unroll
                 for (uint32_t k = 0; k < I; k++) x[k] = dist ? rows[jj[k] * d + (d - dist)] : w[k];
-                seg_lookup_n<I>(g, x, dp.chk_rel[r], keep, sg);
+                const uint2 *ct = r < kDenseChk ? io.chk_seg[r] : nullptr;
+                if (ct) {
+                    const auto tab = gp(reinterpret_cast<const uint64_t *>(ct));
+#pragma unroll
+                    for (uint32_t k = 0; k < I; k++) {
+                        const uint64_t v = keep[k] ? tab[x[k]] : 0ull;
+                        sg[k] = make_uint2((uint32_t)v, (uint32_t)(v >> 32));
+                    }
+                } else {
+                    seg_lookup_n<I>(g, x, dp.chk_rel[r], keep, sg);
+                }
             }
             seg_contains_n<I>(g.nbr, sg, w, keep);
         }
@@ -665,12 +686,13 @@ DEV void expand_tile(const DevGraphView &g, const DP &dp, CP col, CP dcol,
             if (need_nxt && stage_meta) {
 #pragma unroll
                 for (uint32_t k = 0; k < I; k++) ns[k] = s_nxt[jj[k]];
-            } else if (!need_nxt && io.nxt_seg) {
-                // the new vertex's segment: one read of an L2-sized table
-                const auto tab = gp(reinterpret_cast<const uint64_t *>(io.nxt_seg));  // {begin, end} words
+            } else if (io.nxt_seg) {
+                // one read of the relation's dense table ({begin, end} words)
+                const auto tab = gp(reinterpret_cast<const uint64_t *>(io.nxt_seg));
 #pragma unroll
                 for (uint32_t k = 0; k < I; k++) {
-                    const uint64_t v = keep[k] ? tab[w[k]] : 0ull;
+                    const uint32_t x = need_nxt ? rows[jj[k] * d + (d + 1 - dp.nxt_dist)] : w[k];
+                    const uint64_t v = keep[k] ? tab[x] : 0ull;
                     ns[k] = make_uint2((uint32_t)v, (uint32_t)(v >> 32));
                 }
             } else {

@@ -542,12 +542,16 @@ DEV void expand_tile(const DevGraphView &g, const DP &dp, CP col, CP dcol,
     const uint32_t nf = (uint32_t)(sh.node[1] - ilo + 1);
     const uint32_t nchk = dp.nchk;
     const uint32_t need_nxt = (KIND == EXPAND_MID) ? io.need_nxt : 0u;
+    // The rows come first: every candidate reads its node's row (d words,
+    // twice on inner depths), but a node's segments save lookups only for
+    // its candidates after the first.
     const uint32_t meta_words = 2 * nf * nchk + (need_nxt ? 2 * nf : 0);
-    const bool stage_meta = meta_words <= pool_words;
-    const bool staged = stage_meta && meta_words + nf * d <= pool_words;
-    uint2 *s_chk = reinterpret_cast<uint2 *>(s_pool);  // [nchk][nf]
-    uint2 *s_nxt = s_chk + nchk * nf;                   // [nf]
-    uint32_t *s_rows = s_pool + meta_words;             // [nf][d]
+    const uint32_t row_words = (nf * d + 1) & ~1u;  // keeps the segments 8-byte aligned
+    const bool staged = row_words <= pool_words;
+    const bool stage_meta = meta_words + (staged ? row_words : 0) <= pool_words;
+    uint32_t *s_rows = s_pool;                                                   // [nf][d]
+    uint2 *s_chk = reinterpret_cast<uint2 *>(s_pool + (staged ? row_words : 0));  // [nchk][nf]
+    uint2 *s_nxt = s_chk + nchk * nf;                                            // [nf]
     const auto in_rows = gp(io.in_vec) + ilo * d;       // the tile's node rows, contiguous
 
     if (staged)

@@ -32,6 +32,7 @@ DIGEST = 4
 TUNE_PLAIN = 1
 TUNE_BATCH = 2
 PROFILE = 8
+ORDER_COST = 16  # matching order from the data graph's relation statistics
 WALK_DENSE = 0
 WALK_SPARSE = 1
 ERR_PENDING = 6
@@ -120,6 +121,7 @@ def lib():
         "gfsm_graph_free": [vp],
         "gfsm_plan_query": [vp, P(u32), P(u32), pu32, pu32, pu32],
         "gfsm_query_roots": [vp, vp, pu32, P(u64)],
+        "gfsm_plan_order": [vp, vp, u32, pu32],
         "gfsm_query_window": [vp, vp, u64, pu32, pu32, P(u64)],
         "gfsm_match": [vp, vp, vp, P(MatchOpts), P(vp)],
         "gfsm_result_count": [vp, P(u64)],
@@ -284,6 +286,14 @@ class Graph:
                                       C.byref(n)), "roots")
         return ids[: n.value]
 
+    def order(self, query, cost=False):
+        """Matching order (original query ids by depth) of `query` on this data
+        graph: the reference's, or with cost=True the statistics-driven one."""
+        col = np.zeros(query.V, np.uint32)
+        _check(lib().gfsm_plan_order(self._h, query._h, ORDER_COST if cost else 0,
+                                     col.ctypes.data_as(C.POINTER(C.c_uint32))), "plan_order")
+        return col
+
     def window(self, query, max_first):
         """(root_lo, root_hi, first): the highest-id roots of `query` whose
         depth-1 candidates total at most max_first (at least one root)."""
@@ -337,9 +347,9 @@ class Match:
 
 
 def _opts(rows=False, sorted=False, digest=False, profile=False, root_lo=0, root_hi=0, max_rows=0,
-          mem_budget=0, max_cand=0, tile=0, max_depth_cand=0):
+          mem_budget=0, max_cand=0, tile=0, max_depth_cand=0, order_cost=False):
     flags = (ROWS if rows else 0) | (SORTED if sorted else 0) | (DIGEST if digest else 0) | \
-        (PROFILE if profile else 0)
+        (PROFILE if profile else 0) | (ORDER_COST if order_cost else 0)
     return MatchOpts(flags, root_lo, root_hi, tile, max_rows, mem_budget, max_cand, max_depth_cand)
 
 

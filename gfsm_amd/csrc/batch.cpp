@@ -94,28 +94,6 @@ uint64_t RegionAlloc::largest() const
 // memory projection
 // ---------------------------------------------------------------------------
 
-// Per relation of the data graph: vertices owning a segment and entries.
-struct RelStats {
-    std::unordered_map<uint32_t, std::pair<uint64_t, uint64_t>> by_rel;  // rel -> (sources, entries)
-};
-
-static const RelStats &rel_stats(const HostGraph &g)
-{
-    static std::mutex mu;
-    static std::unordered_map<const HostGraph *, std::unique_ptr<RelStats>> cache;
-    std::lock_guard<std::mutex> lk(mu);
-    auto &p = cache[&g];
-    if (!p) {
-        p.reset(new RelStats);
-        for (size_t k = 0; k < g.rel_keys.size(); k++) {
-            uint64_t e = 0;
-            for (uint64_t i = g.rel_off[k]; i < g.rel_off[k + 1]; i++) e += g.rel_cnt[i];
-            p->by_rel[g.rel_keys[k]] = {g.rel_off[k + 1] - g.rel_off[k], e};
-        }
-    }
-    return *p;
-}
-
 static uint64_t node_bytes(uint32_t k) { return 4ull * (k + 1) + 12; }
 
 // Smallest region whose equal per-level share still gives every level
@@ -134,18 +112,18 @@ uint64_t BatchEngine::min_region(uint32_t q) const
 // length of the next generator) children.  Bytes = the levels at those sizes.
 uint64_t BatchEngine::project_bytes(const Plan &plan, uint64_t nroots, uint64_t g1) const
 {
-    const RelStats &rs = rel_stats(data_);
     const uint32_t q = plan.q;
     auto stats = [&](uint32_t rel, double &mean, double &reach) {
         mean = 0;
         reach = 1;
-        auto it = rs.by_rel.find(rel);
-        if (it == rs.by_rel.end() || it->second.first == 0) return;
-        mean = (double)it->second.second / (double)it->second.first;
+        uint64_t src, ent;
+        if (!relation_sources(data_, rel, src, ent) || src == 0) return;
+        mean = (double)ent / (double)src;
         uint32_t rev = rel;
         if (data_.mode == GFSM_MODE_VLABEL) rev = ((rel >> 10) & 0x7FF) << 21 | (rel >> 21) << 10 | (rel & 0x3FF);
-        auto jt = rs.by_rel.find(rev);
-        reach = (double)((jt != rs.by_rel.end()) ? jt->second.first : it->second.first);
+        uint64_t rsrc = src, rent;
+        if (!relation_sources(data_, rev, rsrc, rent)) rsrc = src;
+        reach = (double)rsrc;
         if (reach < 1) reach = 1;
     };
     double S = (double)nroots, G = (double)g1, bytes = 0;
@@ -548,7 +526,7 @@ Status match_batch_shaped(Ctx &ctx, const HostGraph &data, const std::vector<con
         job->id = i;
         job->query = queries[i];
         job->opts = o;
-        Status st = plan_query(*queries[i], job->plan);
+        Status st = plan_for(data, *queries[i], o, job->plan);
         if (!st.good()) {
             out[i].status = st.code;
             continue;

@@ -191,6 +191,18 @@ gfsm_status gfsm_plan_query(const gfsm_graph *query, uint32_t *q, uint32_t *maxv
     GUARD_END
 }
 
+gfsm_status gfsm_plan_order(const gfsm_graph *data, const gfsm_graph *query, uint32_t flags, uint32_t *col)
+{
+    GUARD_BEGIN
+    if (!data || !query || !col) return fail(Status::err(GFSM_ERR_ARG, "null argument"));
+    Plan p;
+    Status st = plan_query(*query->g, p, (flags & GFSM_ORDER_COST) ? data->g : nullptr);
+    if (!st.good()) return fail(st);
+    std::memcpy(col, p.col.data(), p.col.size() * sizeof(uint32_t));
+    return ok();
+    GUARD_END
+}
+
 gfsm_status gfsm_query_roots(const gfsm_graph *data, const gfsm_graph *query, uint32_t *ids, uint64_t *n)
 {
     GUARD_BEGIN

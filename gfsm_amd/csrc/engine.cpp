@@ -240,6 +240,14 @@ Status upload_graph(Ctx &ctx, const HostGraph &g, DevGraph *&out)
     return Status::ok();
 }
 
+Status plan_for(const HostGraph &data, const HostGraph &query, const gfsm_match_opts &o, Plan &plan)
+{
+    if (!(o.flags & GFSM_ORDER_COST)) return plan_query(query, plan);
+    if (o.root_lo || o.root_hi)
+        return Status::err(GFSM_ERR_ARG, "root windows refer to the reference order's root; not with GFSM_ORDER_COST");
+    return plan_query(query, plan, &data);
+}
+
 void root_range(const HostGraph &data, const Plan &plan, uint32_t root_lo, uint32_t root_hi, uint64_t &lo,
                 uint64_t &hi)
 {
@@ -701,7 +709,7 @@ Status match_plain(Ctx &ctx, const HostGraph &data, const HostGraph &query, cons
     res = Result{};
     if (data.mode != query.mode) return Status::err(GFSM_ERR_ARG, "query and data graph use different modes");
     Plan plan;
-    Status st = plan_query(query, plan);
+    Status st = plan_for(data, query, o, plan);
     if (!st.good()) return st;
     res.width = plan.q;
     res.stats.q = plan.q;

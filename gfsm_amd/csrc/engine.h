@@ -105,6 +105,10 @@ void release_graph_mirrors(const HostGraph &g);
 Status match_plain(Ctx &ctx, const HostGraph &data, const HostGraph &query, const gfsm_match_opts &opts,
                    Result &res);
 
+// The query's plan for these options: the reference order, or with
+// GFSM_ORDER_COST the data-statistics order (root windows then rejected).
+Status plan_for(const HostGraph &data, const HostGraph &query, const gfsm_match_opts &o, Plan &plan);
+
 // Matcher helpers shared with the batched and serving paths.
 Status match_single_vertex(const HostGraph &data, const Plan &plan, const gfsm_match_opts &opts, Result &res);
 void root_range(const HostGraph &data, const Plan &plan, uint32_t root_lo, uint32_t root_hi, uint64_t &lo,

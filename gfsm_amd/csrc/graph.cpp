@@ -64,6 +64,24 @@ void HostGraph::segment(uint32_t x, uint32_t rel, uint32_t &beg, uint32_t &end) 
     }
 }
 
+bool relation_sources(const HostGraph &g, uint32_t rel, uint64_t &sources, uint64_t &entries)
+{
+    const int64_t k = g.rel_index(rel);
+    if (k < 0) return false;
+    {
+        std::lock_guard<std::mutex> lk(g.stats_mu);
+        if (g.rel_entries.size() != g.rel_keys.size()) {
+            std::vector<uint64_t> e(g.rel_keys.size(), 0);
+            for (size_t r = 0; r < e.size(); r++)
+                for (uint64_t i = g.rel_off[r]; i < g.rel_off[r + 1]; i++) e[r] += g.rel_cnt[i];
+            g.rel_entries.swap(e);
+        }
+        entries = g.rel_entries[k];
+    }
+    sources = g.rel_off[k + 1] - g.rel_off[k];
+    return true;
+}
+
 int64_t HostGraph::rel_index(uint32_t rel) const
 {
     auto it = std::lower_bound(rel_keys.begin(), rel_keys.end(), rel);

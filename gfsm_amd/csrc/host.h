@@ -68,7 +68,13 @@ struct HostGraph {
 
     mutable std::mutex dev_mu;
     mutable std::unordered_map<const void *, DevGraph *> dev;  // ctx -> mirror
+    mutable std::mutex stats_mu;
+    mutable std::vector<uint64_t> rel_entries;  // per rel_keys index, filled on first use
 };
+
+// Vertices owning a segment of `rel` and the entries of those segments
+// (false when the graph has no such relation).  Computed once per graph.
+bool relation_sources(const HostGraph &g, uint32_t rel, uint64_t &sources, uint64_t &entries);
 
 // Relation of an entry: REF = edge label; VLABEL = (lu << 21) | (lv << 10) | l.
 bool relation_of(uint32_t mode, uint32_t lu, uint32_t lv, uint32_t l, uint32_t &rel);
@@ -89,7 +95,9 @@ struct Plan {
     std::vector<uint32_t> header, requirements;  // reference layout (HnGPU.cuh:41-58)
     QueryPlanDev dev;                       // what the kernels read
 };
-Status plan_query(const HostGraph &query, Plan &out);
+// The reference's matching order, or with `by_cost` (a data graph) the
+// order chosen from its relation statistics.
+Status plan_query(const HostGraph &query, Plan &out, const HostGraph *by_cost = nullptr);
 
 // Workload generators (gen.cpp).
 Status gen_er(uint32_t n, double avg_degree, uint32_t nvl, uint32_t nel, uint64_t seed, uint32_t mode,

@@ -13,6 +13,14 @@
 // The reference generates from every entry of that ancestor's row and then
 // checks all requirements; here the generator's requirement selects the
 // segment that is streamed, and the remaining requirements become checks.
+//
+// With a data graph given (GFSM_ORDER_COST, no reference counterpart) the
+// order is chosen from the data graph's relation statistics instead: the
+// root is the vertex whose cheapest out-relation has the fewest source
+// vertices, and each later depth places the reachable vertex with the
+// smallest expected number of children per node (mean generator segment
+// length times the pass rate of its checks).  Requirements are derived the
+// same way for either order; the set of embeddings does not depend on it.
 #include "host.h"
 
 #include <algorithm>
@@ -21,7 +29,27 @@
 
 namespace gfsm {
 
-Status plan_query(const HostGraph &qg, Plan &out)
+namespace {
+
+// Mean segment length of `rel` in the data graph and the number of vertices
+// its segments can reach (the reverse relation's sources in labelled mode).
+void rel_model(const HostGraph &data, uint32_t rel, double &mean, double &reach)
+{
+    mean = 0;
+    reach = 1;
+    uint64_t src, ent;
+    if (!relation_sources(data, rel, src, ent) || src == 0) return;
+    mean = (double)ent / (double)src;
+    uint32_t rev = rel;
+    if (data.mode == GFSM_MODE_VLABEL) rev = ((rel >> 10) & 0x7FF) << 21 | (rel >> 21) << 10 | (rel & 0x3FF);
+    uint64_t rsrc = src, rent;
+    if (!relation_sources(data, rev, rsrc, rent)) rsrc = src;
+    reach = std::max<double>(1.0, (double)rsrc);
+}
+
+}  // namespace
+
+Status plan_query(const HostGraph &qg, Plan &out, const HostGraph *by_cost)
 {
     out = Plan{};
     uint32_t q = qg.V;
@@ -55,33 +83,87 @@ Status plan_query(const HostGraph &qg, Plan &out)
 
     std::vector<int32_t> placed(q, -1);
     std::vector<char> seen(q, 0);
-    seen[rank[0]] = 1;
     struct Req { uint32_t dist, rel; };
-    for (uint32_t d = 0; d < q; d++) {
-        int64_t n = -1;
-        for (uint32_t i = 0; i < q; i++)
-            if (seen[rank[i]] && placed[rank[i]] < 0) { n = rank[i]; break; }
-        if (n < 0) return Status::err(GFSM_ERR_QUERY, "query is not connected through out-entries");
-        placed[n] = (int32_t)d;
-        out.col[d] = (uint32_t)n;
+    // Requirements of vertex n if it is placed at depth d: each entry n->u to
+    // a placed u (the relation of u->n when the query has it, else of n->u),
+    // by distance.  Placing n itself makes its self loops distance 0.
+    auto reqs_of = [&](uint32_t n, uint32_t d) {
         std::vector<Req> reqs;
         for (uint32_t j = qg.dir_ptr[n]; j < qg.dir_ptr[n + 1]; j++) {
             uint32_t r = qg.dir[j].rel;
             for (uint32_t p = qg.dir[j].beg; p < qg.dir[j + 1].beg; p++) {
                 uint32_t u = qg.nbr[p];
-                seen[u] = 1;
-                if (placed[u] < 0) continue;
+                if (placed[u] < 0 && u != n) continue;
+                const uint32_t du = (u == n) ? d : (uint32_t)placed[u];
                 uint32_t rel = r;
                 for (uint32_t k = qg.dir_ptr[u]; k < qg.dir_ptr[u + 1]; k++) {
                     const uint32_t *b = qg.nbr.data() + qg.dir[k].beg, *e = qg.nbr.data() + qg.dir[k + 1].beg;
                     if (std::binary_search(b, e, (uint32_t)n)) { rel = qg.dir[k].rel; break; }
                 }
-                reqs.push_back({d - (uint32_t)placed[u], rel});
+                reqs.push_back({d - du, rel});
             }
         }
         std::sort(reqs.begin(), reqs.end(), [](const Req &a, const Req &b) {
             return a.dist != b.dist ? a.dist < b.dist : a.rel < b.rel;
         });
+        return reqs;
+    };
+    // Next vertex in the reference order: the lowest-ranked reachable one.
+    auto next_reference = [&](uint32_t d) -> int64_t {
+        if (d == 0) return rank[0];
+        for (uint32_t i = 0; i < q; i++)
+            if (seen[rank[i]] && placed[rank[i]] < 0) return rank[i];
+        return -1;
+    };
+    // Next vertex by expected cost on the data graph (ties: rank order).
+    auto next_by_cost = [&](uint32_t d) -> int64_t {
+        int64_t best = -1;
+        double best_cost = 0;
+        size_t best_reqs = 0;
+        for (uint32_t i = 0; i < q; i++) {
+            const uint32_t n = rank[i];
+            if (placed[n] >= 0 || (d && !seen[n])) continue;
+            double cost;
+            size_t nreq = 0;
+            if (d == 0) {
+                cost = 1e300;  // fewest roots: the cheapest out-relation's sources
+                for (uint32_t j = qg.dir_ptr[n]; j < qg.dir_ptr[n + 1]; j++) {
+                    uint64_t src = 0, ent;
+                    if (!relation_sources(*by_cost, qg.dir[j].rel, src, ent)) src = 0;
+                    cost = std::min(cost, (double)src);
+                }
+            } else {
+                const std::vector<Req> reqs = reqs_of(n, d);
+                bool gen = false;
+                cost = 1;
+                for (auto &r : reqs) {
+                    double mean, reach;
+                    rel_model(*by_cost, r.rel, mean, reach);
+                    if (!gen && r.dist) {
+                        cost *= mean;  // children per node: the generator segment
+                        gen = true;
+                    } else {
+                        cost *= std::min(1.0, mean / reach);  // pass rate of a check
+                    }
+                }
+                if (!gen) continue;  // no placed out-neighbour to generate from
+                nreq = reqs.size();
+            }
+            if (best < 0 || cost < best_cost || (cost == best_cost && nreq > best_reqs)) {
+                best = n;
+                best_cost = cost;
+                best_reqs = nreq;
+            }
+        }
+        return best;
+    };
+    for (uint32_t d = 0; d < q; d++) {
+        const int64_t n = by_cost ? next_by_cost(d) : next_reference(d);
+        if (n < 0) return Status::err(GFSM_ERR_QUERY, "query is not connected through out-entries");
+        const std::vector<Req> reqs = reqs_of((uint32_t)n, d);
+        placed[n] = (int32_t)d;
+        out.col[d] = (uint32_t)n;
+        for (uint32_t p = qg.dir[qg.dir_ptr[n]].beg; p < qg.dir[qg.dir_ptr[n + 1]].beg; p++) seen[qg.nbr[p]] = 1;
         uint32_t firstQ = 0;
         for (auto &r : reqs)
             if (r.dist) { firstQ = r.dist; break; }

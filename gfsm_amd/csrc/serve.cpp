@@ -161,7 +161,7 @@ Status server_submit(Server &s, const HostGraph &query, const gfsm_match_opts *o
     job->query = &query;
     if (opts) job->opts = *opts;
     job->opts.flags |= s.opts.flags;
-    Status plan_st = (query.mode == s.data->mode) ? plan_query(query, job->plan)
+    Status plan_st = (query.mode == s.data->mode) ? plan_for(*s.data, query, job->opts, job->plan)
                                                    : Status::err(GFSM_ERR_ARG, "query and data graph use different modes");
     {
         std::lock_guard<std::mutex> lk(s.mu);

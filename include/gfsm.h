@@ -72,6 +72,10 @@ typedef int32_t gfsm_status;
 #define GFSM_SORTED 2u   /* ... sorted lexicographically by (f(0), ..., f(q-1))    */
 #define GFSM_DIGEST 4u   /* compute the order-independent multiset digest         */
 #define GFSM_PROFILE 8u  /* time every kernel launch with HIP events (stats)      */
+#define GFSM_ORDER_COST 16u /* matching order chosen from the data graph's relation  */
+                            /* statistics instead of the reference's (no reference   */
+                            /* counterpart; the same embeddings, in other row order);*/
+                            /* root_lo/root_hi are then rejected (GFSM_ERR_ARG)       */
 
 #define GFSM_MAX_Q 32u   /* largest query handled (vertices)                      */
 
@@ -162,6 +166,10 @@ gfsm_status gfsm_plan_query(const gfsm_graph *query, uint32_t *q, uint32_t *maxv
  * start an embedding.  Pass ids = NULL to learn n first.  (No counterpart: the
  * reference starts from every row, HnGPU.cu:475-495.) */
 gfsm_status gfsm_query_roots(const gfsm_graph *data, const gfsm_graph *query, uint32_t *ids, uint64_t *n);
+/* Matching order gfsm_match uses for `flags`: original query ids by depth
+ * (col[query size]).  The reference's order, or with GFSM_ORDER_COST the one
+ * chosen from data's relation statistics (no reference counterpart). */
+gfsm_status gfsm_plan_order(const gfsm_graph *data, const gfsm_graph *query, uint32_t flags, uint32_t *col);
 
 /* A root window for bounded work (no counterpart): the largest set of the
  * highest-id root candidates whose depth-1 candidates total at most max_first

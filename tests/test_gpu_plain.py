@@ -120,3 +120,27 @@ def test_root_range_partition(ctx):
     d0 = sum(p.digest[0] for p in parts) % (1 << 64)
     d1 = sum(p.digest[1] for p in parts) % (1 << 64)
     assert (d0, d1) == full.digest
+
+
+@pytest.mark.parametrize("mode", [G.MODE_REF, G.MODE_VLABEL])
+def test_cost_order_same_embeddings(ctx, mode):
+    """GFSM_ORDER_COST plans another order; the embeddings are the same:
+    count and digest against the reference order and the oracle, sorted rows,
+    the batched path, and root windows rejected."""
+    data = G.Graph.er(700, 6.0, nvlabels=3, nelabels=2, seed=41, mode=mode)
+    qs = [G.Graph.walk_query(data, s, kind=s % 2, seed=4100 + s) for s in (3, 4, 5, 6, 7)]
+    for q in qs:
+        a = ctx.match(data, q, digest=True)
+        b = ctx.match(data, q, digest=True, order_cost=True)
+        o = oracle_solve(data, q, mode, rows=False)
+        assert (b.count, b.digest) == (a.count, a.digest) == (o.count, o.digest)
+    q = qs[2]
+    ra = ctx.match(data, q, rows=True, sorted=True)
+    rb = ctx.match(data, q, rows=True, sorted=True, order_cost=True)
+    assert np.array_equal(ra.rows, rb.rows)
+    ms = ctx.match_batch(data, qs, digest=True, order_cost=True)
+    assert [(m.count, m.digest) for m in ms] == [(ctx.match(data, q, digest=True).count,
+                                                   ctx.match(data, q, digest=True).digest) for q in qs]
+    with pytest.raises(G.GfsmError) as ei:
+        ctx.match(data, q, order_cost=True, root_lo=0, root_hi=10)
+    assert ei.value.code == 2

@@ -250,3 +250,45 @@ def test_cli_cache_uses_image_of_same_mode(tmp_path):
     os.utime(data, (later, later))  # text newer than the image: parsed again
     r = subprocess.run([cli, q, str(data), "--cache", str(cache), "--mode", "vlabel"], capture_output=True)
     assert r.returncode == 234
+
+
+def test_cost_order_is_a_connected_permutation():
+    """The statistics-driven order is a permutation in which every later
+    vertex has an out-entry to an earlier one (so it has a generator), and the
+    default order is the reference planner's."""
+    data = G.Graph.rmat(11, 8, nvlabels=4, zipf_s=1.0, nelabels=2, seed=21, mode=G.MODE_VLABEL)
+    differ = 0
+    for seed in range(16):
+        q = G.Graph.walk_query(data, 3 + seed % 8, kind=seed % 2, seed=600 + seed)
+        V, s, d, l, vl = q.edges()
+        ref, cost = data.order(q), data.order(q, cost=True)
+        assert np.array_equal(ref, q.plan()[0])
+        for col in (ref, cost):
+            assert sorted(col.tolist()) == list(range(V))
+            pos = {int(v): i for i, v in enumerate(col)}
+            for i in range(1, V):
+                v = int(col[i])
+                assert any(pos[int(t)] < i for u, t in zip(s, d) if int(u) == v), (seed, col)
+        differ += not np.array_equal(ref, cost)
+    assert differ > 0
+
+
+def test_cost_order_roots_at_the_rarest_relation():
+    """A labelled path whose last relation has one source vertex in the data
+    graph: the statistics-driven order starts at an end of that relation, the
+    reference order at the lowest-id vertex of highest out-degree."""
+    # data: labels 1,1,...; vertex 0 has label 2 and one neighbour; all others label 1
+    nv = 64
+    src, dst = [], []
+    for i in range(1, nv - 1):
+        src += [i, i + 1]
+        dst += [i + 1, i]
+    src += [0, 1]
+    dst += [1, 0]
+    vl = [2] + [1] * (nv - 1)
+    data = G.Graph.from_edges(nv, src, dst, [1] * len(src), vlabel=vl, mode=G.MODE_VLABEL)
+    # query path: 0(label 1) - 1(label 1) - 2(label 1) - 3(label 2)
+    q = G.Graph.from_edges(4, [0, 1, 1, 2, 2, 3], [1, 0, 2, 1, 3, 2], [1] * 6, vlabel=[1, 1, 1, 2],
+                           mode=G.MODE_VLABEL)
+    assert int(data.order(q, cost=True)[0]) in (2, 3)
+    assert int(data.order(q)[0]) == 1

@@ -45,6 +45,29 @@ class GfsmError(RuntimeError):
         self.code = code
 
 
+def read_fs(path):
+    """Rows of an output.fs (little-endian u32 count, u32 width, then count
+    rows of width u32 -- the words the reference's Java checker reads,
+    HnFunction.java:21-38) as a (count, width) uint32 array."""
+    raw = np.fromfile(str(path), dtype="<u4")
+    if raw.size < 2:
+        raise GfsmError(233, f"{path}: shorter than the output.fs header")
+    n, w = int(raw[0]), int(raw[1])
+    if raw.size != 2 + n * w:
+        raise GfsmError(233, f"{path}: header says {n} rows of {w} words, file holds {raw.size - 2} words")
+    return raw[2:].reshape(n, w) if w else np.zeros((n, 0), np.uint32)
+
+
+def format_rows(rows, limit=10):
+    """Embeddings as text, one per line: "q0->f(q0) q1->f(q1) ..." (query
+    vertex -> data vertex), at most `limit` lines plus a count of the rest."""
+    rows = np.asarray(rows)
+    out = [" ".join(f"{c}->{int(v)}" for c, v in enumerate(r)) for r in rows[:limit]]
+    if len(rows) > limit:
+        out.append(f"... {len(rows) - limit} more")
+    return "\n".join(out)
+
+
 class MatchOpts(C.Structure):
     _fields_ = [("flags", C.c_uint32), ("root_lo", C.c_uint32), ("root_hi", C.c_uint32),
                 ("tile", C.c_uint32), ("max_rows", C.c_uint64), ("mem_budget", C.c_uint64),
@@ -130,6 +153,7 @@ def lib():
         "gfsm_result_stats": [vp, P(Stats)],
         "gfsm_result_status": [vp],
         "gfsm_result_write_fs": [vp, C.c_char_p],
+        "gfsm_match_write_fs": [vp, vp, vp, P(MatchOpts), C.c_char_p, P(vp)],
         "gfsm_result_free": [vp],
         "gfsm_row_hash": [pu32, u32, P(u64)],
         "gfsm_match_batch": [vp, vp, P(vp), u32, P(MatchOpts), u32, P(vp)],
@@ -387,6 +411,17 @@ class Context:
 
     def upload(self, g: Graph):
         _check(lib().gfsm_graph_upload(self._h, g._h), "upload")
+
+    def match_to_fs(self, data: Graph, query: Graph, path, allow_partial=False, **kw) -> Match:
+        """Match and write output.fs at `path` while matching (rows streamed
+        chunk by chunk unless sorted=True); the Match carries no rows."""
+        o = _opts(**kw)
+        r = C.c_void_p()
+        rc = lib().gfsm_match_write_fs(self._h, data._h, query._h, C.byref(o), str(path).encode(), C.byref(r))
+        if rc == 5 and r.value and allow_partial:
+            return Match(r)
+        _check(rc, "match_write_fs")
+        return Match(r)
 
     def match(self, data: Graph, query: Graph, allow_partial=False, **kw) -> Match:
         """Match query in data.  With a work cap (max_cand) and allow_partial,

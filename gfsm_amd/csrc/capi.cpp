@@ -289,6 +289,41 @@ gfsm_status gfsm_match(gfsm_ctx *ctx, const gfsm_graph *data, const gfsm_graph *
     GUARD_END
 }
 
+gfsm_status gfsm_match_write_fs(gfsm_ctx *ctx, const gfsm_graph *data, const gfsm_graph *query,
+                                const gfsm_match_opts *opts, const char *path, gfsm_result **r)
+{
+    GUARD_BEGIN
+    if (!ctx || !data || !query || !path || !r) return fail(Status::err(GFSM_ERR_ARG, "null argument"));
+    *r = nullptr;
+    gfsm_match_opts o{};
+    if (opts) o = *opts;
+    o.flags |= GFSM_ROWS;
+    std::unique_ptr<FILE, int (*)(FILE *)> f(std::fopen(path, "wb"), std::fclose);
+    if (!f) return fail(Status::err(GFSM_ERR_MISSING_FILE, std::string("cannot write ") + path));
+    uint32_t hdr[2] = {0, 0};  // rewritten once the count is known
+    if (std::fwrite(hdr, sizeof(uint32_t), 2, f.get()) != 2)
+        return fail(Status::err(GFSM_ERR_MISSING_FILE, "short write to output.fs"));
+    std::unique_ptr<gfsm_result> res(new gfsm_result);
+    // Sorted rows need the whole table; otherwise every chunk is written as it lands.
+    Status st = match_plain(*ctx->c, *data->g, *query->g, o, res->r, (o.flags & GFSM_SORTED) ? nullptr : f.get());
+    const bool partial = st.code == GFSM_ERR_CAP && res->r.status == GFSM_ERR_CAP;
+    if (!st.good() && !partial) return fail(st);
+    Result &m = res->r;
+    const std::vector<uint32_t> &rows = m.rows;  // one-vertex queries and sorted rows
+    if (!rows.empty() && std::fwrite(rows.data(), sizeof(uint32_t), rows.size(), f.get()) != rows.size())
+        return fail(Status::err(GFSM_ERR_MISSING_FILE, "short write to output.fs"));
+    if (m.count > 0xFFFFFFFFull) return fail(Status::err(GFSM_ERR_CAP, "output.fs holds at most 2^32-1 rows"));
+    hdr[0] = (uint32_t)m.count;
+    hdr[1] = m.width;
+    if (std::fseek(f.get(), 0, SEEK_SET) != 0 || std::fwrite(hdr, sizeof(uint32_t), 2, f.get()) != 2 ||
+        std::fflush(f.get()) != 0)
+        return fail(Status::err(GFSM_ERR_MISSING_FILE, "short write to output.fs"));
+    std::vector<uint32_t>().swap(m.rows);  // the rows live in the file
+    *r = res.release();
+    return partial ? fail(st) : ok();
+    GUARD_END
+}
+
 gfsm_status gfsm_result_count(const gfsm_result *r, uint64_t *n)
 {
     if (!r || !n) return fail(Status::err(GFSM_ERR_ARG, "null argument"));

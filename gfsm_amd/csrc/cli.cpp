@@ -12,6 +12,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <vector>
+#include <algorithm>
 
 #include "gfsm.h"
 
@@ -49,6 +51,33 @@ static gfsm_status load_data(const char *path, const char *cache, uint32_t mode,
     return st;
 }
 
+// The first `limit` embeddings on stderr, one per line: "q0->v q1->w ...";
+// from the result when it holds rows, else read back from output.fs.
+static void print_embeddings(const gfsm_result *r, const char *out, uint64_t limit)
+{
+    const uint32_t *rows = nullptr;
+    uint32_t w = 0;
+    uint64_t n = 0;
+    gfsm_result_rows(r, &rows, &w, &n);
+    std::vector<uint32_t> buf;
+    if (!n && out) {
+        FILE *f = std::fopen(out, "rb");
+        uint32_t hdr[2];
+        if (f && std::fread(hdr, sizeof(uint32_t), 2, f) == 2) {
+            w = hdr[1];
+            n = std::min<uint64_t>(hdr[0], limit);
+            buf.resize(n * w);
+            n = w ? std::fread(buf.data(), sizeof(uint32_t) * w, n, f) : 0;
+            rows = buf.data();
+        }
+        if (f) std::fclose(f);
+    }
+    for (uint64_t i = 0; i < n && i < limit; i++) {
+        for (uint32_t c = 0; c < w; c++) std::fprintf(stderr, "%s%u->%u", c ? " " : "", c, rows[i * w + c]);
+        std::fprintf(stderr, "\n");
+    }
+}
+
 static int die(gfsm_status st)
 {
     std::fprintf(stderr, "\nGFSM error %d: %s\n", st, gfsm_last_error());
@@ -60,12 +89,14 @@ int main(int argc, char **argv)
     if (argc < 3) {
         std::fprintf(stderr,
                      "usage: %s <query.g> <data.g> [--mode ref|vlabel] [--out output.fs] [--sorted]\n"
-                     "          [--mem-budget BYTES] [--device N] [--digest] [--cache data.bin]\n",
+                     "          [--mem-budget BYTES] [--device N] [--digest] [--cache data.bin] [--stream] [--print N]\n",
                      argv[0]);
         return 2;
     }
     uint32_t mode = GFSM_MODE_REF;
     const char *out = nullptr, *cache = nullptr;
+    uint64_t print_rows = 0;
+    bool stream = false;
     uint64_t budget = 0;
     int device = 0;
     uint32_t flags = 0;
@@ -78,6 +109,10 @@ int main(int argc, char **argv)
             budget = std::strtoull(argv[++i], nullptr, 10);
         } else if (!std::strcmp(argv[i], "--device") && i + 1 < argc) {
             device = std::atoi(argv[++i]);
+        } else if (!std::strcmp(argv[i], "--stream")) {
+            stream = true;
+        } else if (!std::strcmp(argv[i], "--print") && i + 1 < argc) {
+            print_rows = std::strtoull(argv[++i], nullptr, 10);
         } else if (!std::strcmp(argv[i], "--cache") && i + 1 < argc) {
             cache = argv[++i];
         } else if (!std::strcmp(argv[i], "--digest")) {
@@ -86,7 +121,7 @@ int main(int argc, char **argv)
             flags |= GFSM_SORTED;
         }
     }
-    if (out) flags |= GFSM_ROWS;
+    if (out || print_rows) flags |= GFSM_ROWS;
     std::printf("\n%s,%s,", base_name(argv[1]).c_str(), base_name(argv[2]).c_str());
     const double t_start = now_s();
     // Inputs are parsed and the query compiled before the device is opened,
@@ -116,7 +151,9 @@ int main(int argc, char **argv)
     o.flags = flags;
     gfsm_result *r = nullptr;
     const double t_gpu = now_s();
-    st = gfsm_match(ctx, d, q, &o, &r);
+    // --stream: rows go to output.fs chunk by chunk during the match (the
+    // write is then inside GPU Time, and Solution Write Time is ~0).
+    st = (out && stream) ? gfsm_match_write_fs(ctx, d, q, &o, out, &r) : gfsm_match(ctx, d, q, &o, &r);
     if (st) return die(st);
     const double gpu_s = now_s() - t_gpu;
     gfsm_stats s;
@@ -127,7 +164,7 @@ int main(int argc, char **argv)
     std::printf("n/a,n/a,");                  // Reallocate / ReKernel: no retry exists
     std::printf("%llu,", (unsigned long long)count);
     t = now_s();
-    if (out) {
+    if (out && !stream) {
         st = gfsm_result_write_fs(r, out);
         if (st) return die(st);
     }
@@ -138,6 +175,7 @@ int main(int argc, char **argv)
         gfsm_result_digest(r, dg);
         std::fprintf(stderr, "digest=%016llx%016llx\n", (unsigned long long)dg[0], (unsigned long long)dg[1]);
     }
+    if (print_rows) print_embeddings(r, out, print_rows);
     gfsm_result_free(r);
     gfsm_graph_free(q);
     gfsm_graph_free(d);

@@ -342,6 +342,11 @@ struct Runner {
     uint64_t max_depth_cand = 0;           // deterministic per-depth cap (0 = none)
     uint64_t depth_done[GFSM_MAXQ] = {};   // candidates tested so far per depth (roots: depth 0)
     uint32_t forced_items = 0;  // from gfsm_match_opts.tile (0 = tuned)
+    // Rows streamed to an output.fs as each chunk lands (unsorted rows mode):
+    // host memory stays one chunk, whatever the result size.
+    FILE *stream = nullptr;
+    uint64_t streamed = 0, sdg[2] = {0, 0};
+    std::vector<uint32_t> stage;
 
     Runner(Ctx &c, const HostGraph &d, const DevGraph &g, const Plan &p, uint32_t f, uint64_t mr, Result &r)
         : ctx(c), data(d), dg(g), plan(p), flags(f), max_rows(mr), res(r) {}
@@ -649,8 +654,25 @@ struct Runner {
                     if (!st.good()) return st;
                 }
             } else {
-                if (max_rows && res.rows.size() / q + keep > max_rows)
+                if (max_rows && res.rows.size() / q + streamed + keep > max_rows)
                     return Status::err(GFSM_ERR_CAP, "more rows than max_rows");
+                if (stream) {
+                    stage.resize(keep * q);
+                    if (keep)
+                        HIP_TRY(hipMemcpy(stage.data(), rows_d, keep * q * sizeof(uint32_t), hipMemcpyDeviceToHost));
+                    if (flags & GFSM_DIGEST)
+                        for (uint64_t r = 0; r < keep; r++) {
+                            uint64_t h[2];
+                            row_hash(stage.data() + r * q, q, h);
+                            sdg[0] += h[0];
+                            sdg[1] += h[1];
+                        }
+                    if (keep && std::fwrite(stage.data(), sizeof(uint32_t), stage.size(), stream) != stage.size())
+                        return Status::err(GFSM_ERR_MISSING_FILE, "short write to output.fs");
+                    streamed += keep;
+                    c = e;
+                    continue;
+                }
                 size_t at = res.rows.size();
                 res.rows.resize(at + keep * q);
                 if (keep)
@@ -703,7 +725,8 @@ Status selftest_scan(Ctx &ctx, const uint32_t *vals, uint64_t n, uint64_t *prefi
     return st;
 }
 
-Status match_plain(Ctx &ctx, const HostGraph &data, const HostGraph &query, const gfsm_match_opts &o, Result &res)
+Status match_plain(Ctx &ctx, const HostGraph &data, const HostGraph &query, const gfsm_match_opts &o, Result &res,
+                   FILE *stream)
 {
     const double t0 = now_ms();
     res = Result{};
@@ -725,6 +748,7 @@ Status match_plain(Ctx &ctx, const HostGraph &data, const HostGraph &query, cons
     if (!st.good()) return st;
 
     Runner run(ctx, data, *dg, plan, o.flags, o.max_rows, res);
+    run.stream = stream;
     run.max_cand = o.max_cand;
     run.max_depth_cand = o.max_depth_cand;
     if (o.tile) {
@@ -758,7 +782,12 @@ Status match_plain(Ctx &ctx, const HostGraph &data, const HostGraph &query, cons
     HIP_TRY(hipEventElapsedTime(&ms, L.ev[0], L.ev[1]));
     res.stats.device_ms = ms;
     const uint32_t q = plan.q;
-    if (run.rows_mode) {
+    if (run.rows_mode && stream) {
+        res.count = run.streamed;
+        res.stats.level[q - 1] = res.count;
+        res.digest[0] = run.sdg[0];
+        res.digest[1] = run.sdg[1];
+    } else if (run.rows_mode) {
         res.count = res.rows.size() / q;
         res.stats.level[q - 1] = res.count;
         if (o.flags & GFSM_DIGEST) {

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <atomic>
+#include <cstdio>
 #include <memory>
 #include <mutex>
 #include <set>
@@ -102,8 +103,10 @@ Status upload_graph(Ctx &ctx, const HostGraph &g, DevGraph *&out);
 void release_graph_mirrors(const HostGraph &g);
 
 // Plain single-query path: host-driven depth loop with chunked expansion.
+// With `stream`, rows mode writes each chunk's rows to it (after an output.fs
+// header the caller wrote) instead of keeping them; unsorted only.
 Status match_plain(Ctx &ctx, const HostGraph &data, const HostGraph &query, const gfsm_match_opts &opts,
-                   Result &res);
+                   Result &res, FILE *stream = nullptr);
 
 // The query's plan for these options: the reference order, or with
 // GFSM_ORDER_COST the data-statistics order (root windows then rejected).

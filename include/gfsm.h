@@ -188,6 +188,13 @@ gfsm_status gfsm_result_digest(const gfsm_result *r, uint64_t out[2]);
 gfsm_status gfsm_result_stats(const gfsm_result *r, gfsm_stats *out);
 gfsm_status gfsm_result_status(const gfsm_result *r);
 gfsm_status gfsm_result_write_fs(const gfsm_result *r, const char *path);
+/* Match and write output.fs at `path` (the layout of gfsm_result_write_fs)
+ * while matching: unless GFSM_SORTED is set, each chunk's rows go to the file
+ * as they land, so host memory holds one chunk whatever the result size (no
+ * reference counterpart: the reference copies the whole table first,
+ * HnGPU.cu:1019-1048).  *r holds count, digest and stats, no rows. */
+gfsm_status gfsm_match_write_fs(gfsm_ctx *ctx, const gfsm_graph *data, const gfsm_graph *query,
+                                const gfsm_match_opts *opts, const char *path, gfsm_result **r);
 void        gfsm_result_free(gfsm_result *r);
 
 /* Row digest used by GFSM_DIGEST: h = sum over rows of two seeded hashes of

@@ -35,3 +35,45 @@ def test_cli_matches_oracle(tmp_path, mode, flag):
     rows = np.frombuffer(raw[8:], dtype="<u4").reshape(n, width)
     assert n == ref.count and width == ref.width
     assert np.array_equal(rows, ref.rows)
+
+
+def test_streamed_output_fs_equals_in_memory(ctx, tmp_path):
+    """gfsm_match_write_fs streams each chunk's rows to output.fs: the file
+    holds the oracle's rows (as a set; sorted when asked), also with tiny
+    level buffers (many chunks) and for a one-vertex query."""
+    data = G.Graph.grid(40, 40, del_frac=0.05, diag_frac=0.02, seed=5, mode=G.MODE_REF)
+    query = G.Graph.walk_query(data, 5, kind=G.WALK_SPARSE, seed=17)
+    ref = oracle_solve(data, query, G.MODE_REF)
+    for budget in (0, 1 << 20):
+        p = tmp_path / f"s{budget}.fs"
+        m = ctx.match_to_fs(data, query, p, mem_budget=budget, digest=True)
+        got = G.read_fs(p)
+        assert m.count == ref.count == len(got) and m.digest == ref.digest
+        assert np.array_equal(got[np.lexsort(got.T[::-1])], ref.rows)
+    p = tmp_path / "sorted.fs"
+    ctx.match_to_fs(data, query, p, sorted=True)
+    assert np.array_equal(G.read_fs(p), ref.rows)
+    one = G.Graph.walk_query(data, 1, seed=3)
+    p = tmp_path / "one.fs"
+    m1 = ctx.match_to_fs(data, one, p)
+    full = ctx.match(data, one, rows=True, sorted=True)  # one-vertex queries: the oracle has none
+    assert np.array_equal(np.sort(G.read_fs(p)[:, 0]), full.rows[:, 0]) and m1.count == full.count
+
+
+def test_cli_stream_and_print(tmp_path):
+    data = G.Graph.er(500, 5.0, nvlabels=2, seed=77, mode=G.MODE_VLABEL)
+    query = G.Graph.walk_query(data, 3, seed=78)
+    dp, qp = tmp_path / "data.g", tmp_path / "query.g"
+    data.write(dp)
+    query.write(qp)
+    a, b = tmp_path / "a.fs", tmp_path / "b.fs"
+    r1 = subprocess.run([CLI, str(qp), str(dp), "--mode", "vlabel", "--out", str(a)], capture_output=True,
+                        text=True, timeout=120)
+    r2 = subprocess.run([CLI, str(qp), str(dp), "--mode", "vlabel", "--out", str(b), "--stream", "--print", "3"],
+                        capture_output=True, text=True, timeout=120)
+    assert r1.returncode == 0 and r2.returncode == 0, (r1.stderr, r2.stderr)
+    ra, rb = G.read_fs(a), G.read_fs(b)
+    key = lambda x: x[np.lexsort(x.T[::-1])]
+    assert len(ra) == len(rb) > 3 and np.array_equal(key(ra), key(rb))
+    lines = [l for l in r2.stderr.splitlines() if "->" in l]
+    assert lines == [G.format_rows(rb[i:i + 1]) for i in range(3)]

@@ -292,3 +292,21 @@ def test_cost_order_roots_at_the_rarest_relation():
                            mode=G.MODE_VLABEL)
     assert int(data.order(q, cost=True)[0]) in (2, 3)
     assert int(data.order(q)[0]) == 1
+
+
+def test_read_fs_reads_the_oracle_layout_and_rejects_damage(tmp_path):
+    rows = np.array([[3, 1, 4], [1, 5, 9], [2, 6, 5]], dtype=np.uint32)
+    p = tmp_path / "output.fs"
+    O.fs_write(p, rows)
+    assert np.array_equal(G.read_fs(p), rows)
+    raw = p.read_bytes()
+    (tmp_path / "short.fs").write_bytes(raw[:-4])
+    (tmp_path / "tiny.fs").write_bytes(raw[:4])
+    for name in ("short.fs", "tiny.fs"):
+        with pytest.raises(G.GfsmError) as ei:
+            G.read_fs(tmp_path / name)
+        assert ei.value.code == 233
+    empty = tmp_path / "empty.fs"
+    O.fs_write(empty, np.zeros((0, 4), np.uint32))
+    assert G.read_fs(empty).shape == (0, 4)
+    assert G.format_rows(rows, limit=2) == "0->3 1->1 2->4\n0->1 1->5 2->9\n... 1 more"

@@ -317,6 +317,9 @@ constexpr uint64_t kCapStartDiv = 8;
 // Device bytes of dense segment tables one resident graph may hold.
 constexpr uint64_t kDenseMaxBytes = 4ull << 30;
 
+// Candidates per thread of the lean last-depth kernel (its registers fit 64).
+constexpr uint32_t kBareItems = 4;
+
 // One plain match in flight: level buffers carved from the arena, the lane,
 // and the depth-first chunk recursion.  Level k (nodes with k+1 placed
 // vertices) holds rows of width k+1, candidate offsets and generator begins.
@@ -551,10 +554,17 @@ struct Runner {
     Status descend(uint32_t d, uint64_t g0, uint64_t g1, uint64_t S_prev)
     {
         const bool last = (d + 1 == q);
-        const uint32_t kind = last ? (rows_mode ? EXPAND_LAST_ROWS : EXPAND_LAST_COUNT) : EXPAND_MID;
+        // A counting last depth with nothing to check and no digest runs the
+        // lean kernel (at most 4 candidates per thread there: it keeps its
+        // registers under 64 for eight waves per SIMD).
+        const bool bare = last && !rows_mode && !digest && plan.dev.depth[d].nchk == 0 &&
+                          !(forced_items > kBareItems);
+        const uint32_t kind =
+            last ? (rows_mode ? EXPAND_LAST_ROWS : (bare ? EXPAND_LAST_BARE : EXPAND_LAST_COUNT)) : EXPAND_MID;
         const uint64_t tile_limit = launch_limit();
-        const uint32_t items = pick_items(kind, d);
-        const uint32_t pool_words = ctx.tune.pool_words;
+        const uint32_t items = bare ? std::min(pick_items(EXPAND_LAST_COUNT, d), kBareItems) : pick_items(kind, d);
+        const uint32_t pool_words = bare ? std::min(ctx.tune.pool_words, kBarePoolWords) : ctx.tune.pool_words;
+        const uint32_t grid_mul = bare ? 2 : 1;  // twice the resident workgroups
         const uint64_t tc = (uint64_t)kTile * items;
         bool first = true;
         for (uint64_t c = g0; c < g1;) {
@@ -614,13 +624,14 @@ struct Runner {
             }
             HIP_TRY(ctx.lane.before_launch());
             kbegin();
-            HIP_TRY(launch_expand(a, grid_for(io.ntiles), ctx.lane.stream));
+            HIP_TRY(launch_expand(a, (uint32_t)std::min<uint64_t>(io.ntiles, (uint64_t)grid_cap * grid_mul),
+                                  ctx.lane.stream));
             res.stats.launches++;
             if (!first) res.stats.chunks++;
             first = false;
             Status st = kend(true);
             if (!st.good()) return st;
-            if (kind == EXPAND_LAST_COUNT) {
+            if (kind == EXPAND_LAST_COUNT || kind == EXPAND_LAST_BARE) {
                 res.stats.cand[d] += e - c;
                 depth_done[d] += e - c;
                 c = e;

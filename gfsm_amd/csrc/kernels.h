@@ -57,7 +57,10 @@ enum ExpandKind : uint32_t {
     EXPAND_MID = 0,        // keep survivors with children, write the next level
     EXPAND_LAST_COUNT = 1, // last depth: count (and digest)
     EXPAND_LAST_ROWS = 2,  // last depth: write rows in original ids (and digest)
+    EXPAND_LAST_BARE = 3,  // last depth, count only, no checks and no digest: a lean kernel
+                           // that fits twice the resident workgroups of the others
 };
+constexpr uint32_t kBarePoolWords = 2048;  // LDS pool of the lean last-depth kernel
 
 // Data movement of one expand step over a candidate range of depth d.
 // A frontier level holding nodes with k+1 placed vertices (depths 0..k):

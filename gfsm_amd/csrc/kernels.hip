@@ -508,6 +508,8 @@ DEV void expand_tile(const DevGraphView &g, const DP &dp, CP col, CP dcol,
                      uint64_t &acc_h0, uint64_t &acc_h1, bool tickets = true)
 {
     constexpr uint32_t T = kTile, TC = kTile * I, NW = TC / 64;
+    constexpr bool BARE = KIND == EXPAND_LAST_BARE;  // counting, no checks, no digest
+    constexpr bool COUNTING = KIND == EXPAND_LAST_COUNT || BARE;
     const uint32_t d = io.d;
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = lane_id(), wave = tid >> 6;
@@ -540,7 +542,7 @@ DEV void expand_tile(const DevGraphView &g, const DP &dp, CP col, CP dcol,
     if (KIND == EXPAND_LAST_COUNT && tickets) ticket_prefetch(ws, &sh.next);  // no look-back in this kind
     const uint64_t ilo = sh.node[0];
     const uint32_t nf = (uint32_t)(sh.node[1] - ilo + 1);
-    const uint32_t nchk = dp.nchk;
+    const uint32_t nchk = BARE ? 0u : dp.nchk;
     const uint32_t need_nxt = (KIND == EXPAND_MID) ? io.need_nxt : 0u;
     // The rows come first: every candidate reads its node's row (d words,
     // twice on inner depths), but a node's segments save lookups only for
@@ -714,7 +716,7 @@ DEV void expand_tile(const DevGraphView &g, const DP &dp, CP col, CP dcol,
             }
         }
 
-        if (KIND != EXPAND_LAST_COUNT) {
+        if (!COUNTING) {
             // Compaction: wave-level ballot/mbcnt and DPP scans, then the
             // tile's prefix from the look-back, then each item's slot.
             ScanSmem<I> &sm = sh.sm;
@@ -809,7 +811,7 @@ DEV void expand_tile(const DevGraphView &g, const DP &dp, CP col, CP dcol,
 #pragma unroll
             for (uint32_t k = 0; k < I; k++) acc_cnt += keep[k] ? 1u : 0u;
         }
-        if (KIND != EXPAND_MID && io.digest) {
+        if (KIND != EXPAND_MID && !BARE && io.digest) {
 #pragma unroll
             for (uint32_t k = 0; k < I; k++) {
                 if (!keep[k]) continue;
@@ -862,6 +864,25 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(4))) void
         }
     }
     if (KIND != EXPAND_MID) flush_sums(acc_cnt, acc_h0, acc_h1, a.io.tot, sh.red);
+    finish_launch(a.ws);
+}
+
+// The lean last-depth kernel: counting only, no checks, no digest, a small
+// staging pool.  Its tile body needs few registers, so eight waves per SIMD
+// (twice the others) hide the latency of each tile's dependent loads.
+template <uint32_t I>
+__global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(8))) void k_expand_bare(ExpandArgs a)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ TileShared<I> sh;
+    const uint32_t ep = read_epoch(a.ws);
+    uint64_t acc_cnt = 0, acc_h0 = 0, acc_h1 = 0;
+    for (uint64_t t = blockIdx.x; t < a.io.ntiles; t += gridDim.x) {  // round-robin, as for LAST_COUNT
+        __syncthreads();
+        expand_tile<EXPAND_LAST_BARE, I>(a.g, a.dp, a.col, a.dcol, a.io, t, t, 0, a.ws, ep, a.pool_words, sh, smem,
+                                         acc_cnt, acc_h0, acc_h1, false);
+    }
+    flush_sums(acc_cnt, 0, 0, a.io.tot, sh.red);
     finish_launch(a.ws);
 }
 
@@ -1328,6 +1349,29 @@ static hipError_t launch_expand_t(const ExpandArgs &a, uint32_t grid, hipStream_
     return hipGetLastError();
 }
 
+template <uint32_t I>
+static hipError_t launch_expand_bare_t(const ExpandArgs &a, uint32_t grid, hipStream_t s)
+{
+    static bool attr = false;
+    const size_t lds = expand_lds_bytes(I, a.pool_words);
+    if (lds > kMaxDynLds) return hipErrorInvalidValue;
+    hipError_t e = allow_lds(&k_expand_bare<I>, attr);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_expand_bare<I>), dim3(grid), dim3(kTile), lds, s, a);
+    return hipGetLastError();
+}
+
+static hipError_t launch_expand_bare(const ExpandArgs &a, uint32_t grid, hipStream_t s)
+{
+    switch (a.items) {
+    case 1: return launch_expand_bare_t<1>(a, grid, s);
+    case 2: return launch_expand_bare_t<2>(a, grid, s);
+    case 4: return launch_expand_bare_t<4>(a, grid, s);
+    case 8: return launch_expand_bare_t<8>(a, grid, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
 template <uint32_t KIND>
 static hipError_t launch_expand_k(const ExpandArgs &a, uint32_t grid, hipStream_t s)
 {
@@ -1345,7 +1389,9 @@ hipError_t launch_expand(const ExpandArgs &a, uint32_t grid, hipStream_t s)
     switch (a.io.kind) {
     case EXPAND_MID: return launch_expand_k<EXPAND_MID>(a, grid, s);
     case EXPAND_LAST_COUNT: return launch_expand_k<EXPAND_LAST_COUNT>(a, grid, s);
-    default: return launch_expand_k<EXPAND_LAST_ROWS>(a, grid, s);
+    case EXPAND_LAST_ROWS: return launch_expand_k<EXPAND_LAST_ROWS>(a, grid, s);
+    case EXPAND_LAST_BARE: return launch_expand_bare(a, grid, s);
+    default: return hipErrorInvalidValue;
     }
 }
 

@@ -144,3 +144,20 @@ def test_cost_order_same_embeddings(ctx, mode):
     with pytest.raises(G.GfsmError) as ei:
         ctx.match(data, q, order_cost=True, root_lo=0, root_hi=10)
     assert ei.value.code == 2
+
+
+def test_lean_last_depth_counts(ctx):
+    """Count-only matches whose last depth has no checks run the lean
+    last-depth kernel; their counts equal the digest runs (general kernel)
+    and the oracle, for every tile size and with tiny level buffers."""
+    cases = [(G.Graph.er(900, 7.0, nvlabels=2, nelabels=1, seed=61, mode=G.MODE_VLABEL), G.MODE_VLABEL),
+             (G.Graph.rmat(11, 8, nvlabels=3, zipf_s=1.0, nelabels=1, seed=62, mode=G.MODE_VLABEL), G.MODE_VLABEL),
+             (G.Graph.grid(50, 50, del_frac=0.05, diag_frac=0.02, seed=63), G.MODE_REF)]
+    for data, mode in cases:
+        for s in (2, 3, 4, 5, 6):
+            q = G.Graph.walk_query(data, s, kind=G.WALK_SPARSE, seed=6300 + s)
+            ref = oracle_solve(data, q, mode, rows=False).count
+            for tile in (0, 256, 512, 1024, 2048):
+                assert ctx.match(data, q, tile=tile).count == ref, (s, tile)
+            assert ctx.match(data, q, digest=True).count == ref
+            assert ctx.match(data, q, mem_budget=2 << 20).count == ref

@@ -123,19 +123,22 @@ def batch_section():
     ml = [l for _, l in caps]
     ctx = G.Context(0, mem_budget=64 << 30)
     ctx.upload(data)
-    # autotune the batch shapes on the first 64 queries (with their windows and caps)
-    tuned = ctx.autotune(data, qs[:64], windows=wins[:64], caps=mc[:64], depth_caps=ml[:64], which=G.TUNE_BATCH)
+    # autotune the batch shapes on every 4th query (64 of them, with their windows and caps)
+    tuned = ctx.autotune(data, qs[::4], windows=wins[::4], caps=mc[::4], depth_caps=ml[::4], which=G.TUNE_BATCH)
     ctx.match_batch(data, qs[:16], windows=wins[:16], caps=mc[:16], depth_caps=ml[:16])  # engine set-up
     for i in range(16):  # single-query arena set-up, likewise outside the timing
         ctx.match(data, qs[i], root_lo=wins[i][0], root_hi=wins[i][1], max_cand=mc[i], max_depth_cand=ml[i],
                   allow_partial=True)
-    t = time.perf_counter()
-    ser = [ctx.match(data, q, root_lo=lo, root_hi=hi, max_cand=c, max_depth_cand=l, allow_partial=True, digest=True)
-           for q, (lo, hi), c, l in zip(qs, wins, mc, ml)]
-    t_ser = time.perf_counter() - t
-    t = time.perf_counter()
-    bat = ctx.match_batch(data, qs, windows=wins, caps=mc, depth_caps=ml, digest=True)
-    t_bat = time.perf_counter() - t
+    # each side timed twice, the faster pass kept (both are whole passes over the 256 queries)
+    t_ser = t_bat = float("inf")
+    for _ in range(2):
+        t = time.perf_counter()
+        ser = [ctx.match(data, q, root_lo=lo, root_hi=hi, max_cand=c, max_depth_cand=l, allow_partial=True,
+                         digest=True) for q, (lo, hi), c, l in zip(qs, wins, mc, ml)]
+        t_ser = min(t_ser, time.perf_counter() - t)
+        t = time.perf_counter()
+        bat = ctx.match_batch(data, qs, windows=wins, caps=mc, depth_caps=ml, digest=True)
+        t_bat = min(t_bat, time.perf_counter() - t)
     equal = sum((b.count, b.digest, b.status) == (s_.count, s_.digest, s_.status) for b, s_ in zip(bat, ser))
     emb = sum(m.count for m in bat)
     ctx.close()

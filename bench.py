@@ -314,7 +314,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": "gfsm::k_expand",
+            "kernel": "gfsm::k_expand (inner depths) + gfsm::k_expand_bare (last depth)",
             "launches": launches // args.steps,
             "mean_launch_ms": mean_ms,
         },

@@ -77,3 +77,27 @@ def test_cli_stream_and_print(tmp_path):
     assert len(ra) == len(rb) > 3 and np.array_equal(key(ra), key(rb))
     lines = [l for l in r2.stderr.splitlines() if "->" in l]
     assert lines == [G.format_rows(rb[i:i + 1]) for i in range(3)]
+
+
+def test_binary_image_matches_like_the_text_graph(ctx, tmp_path):
+    """A data graph loaded from its binary image matches exactly like the
+    graph parsed from text, through the library and through GFSM --cache."""
+    data = G.Graph.rmat(11, 6, nvlabels=3, zipf_s=1.0, nelabels=2, seed=88, mode=G.MODE_VLABEL)
+    dp, qp, img = tmp_path / "data.g", tmp_path / "query.g", tmp_path / "data.bin"
+    data.write(dp)
+    text = G.Graph.load(dp, mode=G.MODE_VLABEL)
+    text.save_bin(img)
+    image = G.Graph.load_bin(img)
+    for s in (3, 4, 5, 6):
+        q = G.Graph.walk_query(text, s, seed=880 + s)
+        a, b = ctx.match(text, q, digest=True), ctx.match(image, q, digest=True)
+        assert (a.count, a.digest) == (b.count, b.digest)
+    q = G.Graph.walk_query(text, 4, seed=900)
+    q.write(qp)
+    cache = tmp_path / "cache.bin"
+    runs = [subprocess.run([CLI, str(qp), str(dp), "--mode", "vlabel", "--cache", str(cache)], capture_output=True,
+                           text=True, timeout=120) for _ in range(2)]
+    assert all(r.returncode == 0 for r in runs), [r.stderr for r in runs]
+    assert cache.exists()
+    counts = [r.stdout.strip().split(",")[8] for r in runs]  # "Function Count" column
+    assert counts[0] == counts[1] == str(ctx.match(text, q).count)

@@ -204,7 +204,7 @@ Status load_bin(const char *path, HostGraph *&out)
     if (std::fseek(f.get(), 0, SEEK_END) != 0) return Status::err(GFSM_ERR_INVALID_DATA, "cannot size graph image");
     const long long len = std::ftell(f.get());
     if (len < 0 || (uint64_t)len != want) return Status::err(GFSM_ERR_INVALID_DATA, "graph image size mismatch");
-    std::fseek(f.get(), (long)sizeof(h), SEEK_SET);
+    if (std::fseek(f.get(), (long)sizeof(h), SEEK_SET) != 0) return Status::err(GFSM_ERR_INVALID_DATA, "cannot read graph image");
     std::unique_ptr<HostGraph> g(new HostGraph);
     g->mode = h.mode;
     g->V = h.V;

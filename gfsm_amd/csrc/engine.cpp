@@ -317,6 +317,9 @@ constexpr uint64_t kCapStartDiv = 8;
 // Device bytes of dense segment tables one resident graph may hold.
 constexpr uint64_t kDenseMaxBytes = 4ull << 30;
 
+// Steps of at least this many tiles get their frontier nodes from k_tile_index.
+constexpr uint64_t kTileIndexMin = 64;
+
 // Candidates per thread of the lean last-depth kernel (its registers fit 64).
 constexpr uint32_t kBareItems = 4;
 
@@ -618,7 +621,10 @@ struct Runner {
             if (max_cand && cand_total + (e - c) > max_cand)
                 return Status::err(GFSM_ERR_CAP, "candidate cap reached");
             cand_total += e - c;
-            if (ctx.lane.tile_node) {  // the tiles' frontier nodes, searched by the whole GPU ahead of the step
+            // The tiles' frontier nodes, searched by the whole GPU ahead of the
+            // step; a step of a few tiles searches inside its tiles instead of
+            // paying for another launch.
+            if (ctx.lane.tile_node && io.ntiles >= kTileIndexMin) {
                 HIP_TRY(launch_tile_index(io.in_off, S_prev, c, e, tc, io.ntiles, ctx.lane.tile_node, ctx.lane.stream));
                 io.tile_node = ctx.lane.tile_node;
             }

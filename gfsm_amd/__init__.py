@@ -145,6 +145,7 @@ def lib():
         "gfsm_plan_query": [vp, P(u32), P(u32), pu32, pu32, pu32],
         "gfsm_query_roots": [vp, vp, pu32, P(u64)],
         "gfsm_plan_order": [vp, vp, u32, pu32],
+        "gfsm_query_projection": [vp, vp, P(MatchOpts), P(C.c_double), P(C.c_double)],
         "gfsm_query_window": [vp, vp, u64, pu32, pu32, P(u64)],
         "gfsm_match": [vp, vp, vp, P(MatchOpts), P(vp)],
         "gfsm_result_count": [vp, P(u64)],
@@ -309,6 +310,16 @@ class Graph:
         _check(lib().gfsm_query_roots(self._h, query._h, ids.ctypes.data_as(C.POINTER(C.c_uint32)),
                                       C.byref(n)), "roots")
         return ids[: n.value]
+
+    def projection(self, query, **kw):
+        """(nodes, candidates) expected per depth of `query` on this data graph
+        (options as for match: root_lo/root_hi, order_cost)."""
+        o = _opts(**kw)
+        nodes, cand = np.zeros(query.V), np.zeros(query.V)
+        _check(lib().gfsm_query_projection(self._h, query._h, C.byref(o),
+                                           nodes.ctypes.data_as(C.POINTER(C.c_double)),
+                                           cand.ctypes.data_as(C.POINTER(C.c_double))), "projection")
+        return nodes, cand
 
     def order(self, query, cost=False):
         """Matching order (original query ids by depth) of `query` on this data

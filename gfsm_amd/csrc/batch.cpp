@@ -105,49 +105,40 @@ uint64_t BatchEngine::min_region(uint32_t q) const
     return round_up(per_level * (q - 1), kRegionAlign);
 }
 
-// Expected level sizes from relation statistics: a candidate from a segment
-// of relation r passes a check of relation c with probability
-// (mean segment length of c) / (vertices that c can reach), approximated by
-// the reverse relation's source count; each survivor has (mean segment
-// length of the next generator) children.  Bytes = the levels at those sizes.
+// Model bytes of a query's levels (project_levels at the level layout's
+// node sizes, the last level excluded: counting needs no buffer).
+double BatchEngine::model_bytes(const Plan &plan, uint64_t nroots, uint64_t g1) const
+{
+    double nodes[GFSM_MAXQ], cand[GFSM_MAXQ];
+    project_levels(data_, plan, nroots, g1, nodes, cand);
+    double bytes = 0;
+    for (uint32_t k = 0; k + 1 < plan.q; k++) bytes += nodes[k] * node_bytes(k);
+    return bytes;
+}
+
+// Region bytes a query is given: the model scaled by the calibration learnt
+// from finished queries, with headroom, clamped to [min, max] region.
 uint64_t BatchEngine::project_bytes(const Plan &plan, uint64_t nroots, uint64_t g1) const
 {
-    const uint32_t q = plan.q;
-    auto stats = [&](uint32_t rel, double &mean, double &reach) {
-        mean = 0;
-        reach = 1;
-        uint64_t src, ent;
-        if (!relation_sources(data_, rel, src, ent) || src == 0) return;
-        mean = (double)ent / (double)src;
-        uint32_t rev = rel;
-        if (data_.mode == GFSM_MODE_VLABEL) rev = ((rel >> 10) & 0x7FF) << 21 | (rel >> 21) << 10 | (rel & 0x3FF);
-        uint64_t rsrc = src, rent;
-        if (!relation_sources(data_, rev, rsrc, rent)) rsrc = src;
-        reach = (double)rsrc;
-        if (reach < 1) reach = 1;
-    };
-    double S = (double)nroots, G = (double)g1, bytes = 0;
-    bytes += S * node_bytes(0);
-    for (uint32_t d = 1; d + 1 < q; d++) {
-        const DepthPlanDev &dp = plan.dev.depth[d];
-        double p = 1;
-        for (uint32_t r = 0; r < dp.nchk; r++) {
-            double m, reach;
-            stats(dp.chk_rel[r], m, reach);
-            p *= std::min(1.0, m / reach);
-        }
-        S = G * p;
-        double m, reach;
-        stats(dp.nxt_rel, m, reach);
-        G = S * m;
-        bytes += S * node_bytes(d);
-        if (bytes > 1e18) break;
-    }
-    const double b = bytes * 1.25;
-    const uint64_t lo = min_region(q);
+    const double b = model_bytes(plan, nroots, g1) * calib_ * 1.25;
+    const uint64_t lo = min_region(plan.q);
     const uint64_t hi = std::max(lo, max_region_);
     if (!(b < (double)hi)) return hi;
     return std::max(lo, round_up((uint64_t)b, kRegionAlign));
+}
+
+// Calibration: the geometric mean (exponentially weighted) of actual over
+// model level bytes of finished, uncapped queries with non-trivial levels.
+void BatchEngine::calibrate(double model, const Result &r, uint32_t q)
+{
+    if (r.status != GFSM_OK || model < 4096) return;
+    double actual = 0;
+    for (uint32_t k = 0; k + 1 < q; k++) actual += (double)r.stats.level[k] * node_bytes(k);
+    if (actual < 4096) return;
+    const double ratio = std::min(256.0, std::max(1.0 / 256, actual / model));
+    calib_log_ = 0.9 * calib_log_ + 0.1 * std::log(ratio);
+    calib_ = std::exp(calib_log_);
+    calib_n_++;
 }
 
 // ---------------------------------------------------------------------------
@@ -337,6 +328,7 @@ bool BatchEngine::try_admit(std::unique_ptr<BatchJob> &job, Status &st)
     // chunks), more when the projection asks for it, at most max_region.
     const uint64_t share = arena_bytes_ / shape_.slots / kRegionAlign * kRegionAlign;
     const uint64_t cap = std::min(max_region_, arena_bytes_);
+    job->model_bytes = model_bytes(plan, rhi - rlo, g1);
     uint64_t want = std::min(cap, std::max(share, project_bytes(plan, rhi - rlo, g1)));
     want = std::max(lo, want);
     const uint64_t off = regions_.take(want);
@@ -464,6 +456,7 @@ Status BatchEngine::harvest(const std::function<void(std::unique_ptr<BatchJob>, 
         if (b.state == SLOT_ERROR) r.status = GFSM_ERR_ARG;
         regions_.give(job->region_off, job->region_bytes);
         running_--;
+        calibrate(job->model_bytes, r, q);
         done(std::move(job), r);
     }
     return Status::ok();

@@ -203,6 +203,29 @@ gfsm_status gfsm_plan_order(const gfsm_graph *data, const gfsm_graph *query, uin
     GUARD_END
 }
 
+gfsm_status gfsm_query_projection(const gfsm_graph *data, const gfsm_graph *query, const gfsm_match_opts *opts,
+                                  double *nodes, double *cand)
+{
+    GUARD_BEGIN
+    if (!data || !query || !nodes || !cand) return fail(Status::err(GFSM_ERR_ARG, "null argument"));
+    gfsm_match_opts o{};
+    if (opts) o = *opts;
+    Plan p;
+    Status st = plan_for(*data->g, *query->g, o, p);
+    if (!st.good()) return fail(st);
+    if (p.q < 2) return fail(Status::err(GFSM_ERR_ARG, "a one-vertex query has no levels to project"));
+    uint64_t lo, hi, g1 = 0;
+    root_range(*data->g, p, o.root_lo, o.root_hi, lo, hi);
+    for (uint64_t i = lo; i < hi; i++) g1 += data->g->rel_cnt[i];
+    std::vector<double> nd(p.q), cd(p.q);
+    project_levels(*data->g, p, hi - lo, g1, nd.data(), cd.data());
+    // reported by original query depth order: entry k is depth k
+    std::memcpy(nodes, nd.data(), p.q * sizeof(double));
+    std::memcpy(cand, cd.data(), p.q * sizeof(double));
+    return ok();
+    GUARD_END
+}
+
 gfsm_status gfsm_query_roots(const gfsm_graph *data, const gfsm_graph *query, uint32_t *ids, uint64_t *n)
 {
     GUARD_BEGIN

@@ -82,6 +82,19 @@ bool relation_sources(const HostGraph &g, uint32_t rel, uint64_t &sources, uint6
     return true;
 }
 
+uint64_t relation_pool(const HostGraph &g, uint32_t rel)
+{
+    if (g.mode != GFSM_MODE_VLABEL) return g.V;
+    const uint32_t lu = rel >> 21;
+    std::lock_guard<std::mutex> lk(g.stats_mu);
+    if (g.label_count.empty()) {
+        g.label_count.assign(2048, 0);
+        for (uint32_t x = 0; x < g.V; x++)
+            if (g.vlabel[x] < 2048) g.label_count[g.vlabel[x]]++;
+    }
+    return lu < g.label_count.size() ? g.label_count[lu] : 0;
+}
+
 int64_t HostGraph::rel_index(uint32_t rel) const
 {
     auto it = std::lower_bound(rel_keys.begin(), rel_keys.end(), rel);

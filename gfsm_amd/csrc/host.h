@@ -70,7 +70,12 @@ struct HostGraph {
     mutable std::unordered_map<const void *, DevGraph *> dev;  // ctx -> mirror
     mutable std::mutex stats_mu;
     mutable std::vector<uint64_t> rel_entries;  // per rel_keys index, filled on first use
+    mutable std::vector<uint64_t> label_count;  // vertices per vertex label, filled on first use
 };
+
+// Vertices that could own a segment of `rel`: those with the relation's
+// source label in labelled mode, all vertices otherwise.
+uint64_t relation_pool(const HostGraph &g, uint32_t rel);
 
 // Vertices owning a segment of `rel` and the entries of those segments
 // (false when the graph has no such relation).  Computed once per graph.
@@ -98,6 +103,13 @@ struct Plan {
 // The reference's matching order, or with `by_cost` (a data graph) the
 // order chosen from its relation statistics.
 Status plan_query(const HostGraph &query, Plan &out, const HostGraph *by_cost = nullptr);
+// Relation model of the data graph: mean segment length of `rel` and the
+// number of vertices its segments reach.
+void rel_model(const HostGraph &data, uint32_t rel, double &mean, double &reach);
+// Expected nodes and candidates per depth (q entries each) of a plan whose
+// root range has `nroots` roots and `g1` depth-1 candidates.
+void project_levels(const HostGraph &data, const Plan &plan, uint64_t nroots, uint64_t g1, double *nodes,
+                    double *cand);
 
 // Workload generators (gen.cpp).
 Status gen_er(uint32_t n, double avg_degree, uint32_t nvl, uint32_t nel, uint64_t seed, uint32_t mode,

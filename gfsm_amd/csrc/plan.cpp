@@ -29,17 +29,16 @@
 
 namespace gfsm {
 
-namespace {
-
-// Mean segment length of `rel` in the data graph and the number of vertices
-// its segments can reach (the reverse relation's sources in labelled mode).
+// Mean segment length of `rel` over every vertex that could own one (those
+// without a segment count as length 0) and the number of vertices its
+// segments can reach (the reverse relation's sources in labelled mode).
 void rel_model(const HostGraph &data, uint32_t rel, double &mean, double &reach)
 {
     mean = 0;
     reach = 1;
     uint64_t src, ent;
     if (!relation_sources(data, rel, src, ent) || src == 0) return;
-    mean = (double)ent / (double)src;
+    mean = (double)ent / (double)std::max<uint64_t>(src, relation_pool(data, rel));
     uint32_t rev = rel;
     if (data.mode == GFSM_MODE_VLABEL) rev = ((rel >> 10) & 0x7FF) << 21 | (rel >> 21) << 10 | (rel & 0x3FF);
     uint64_t rsrc = src, rent;
@@ -47,7 +46,34 @@ void rel_model(const HostGraph &data, uint32_t rel, double &mean, double &reach)
     reach = std::max<double>(1.0, (double)rsrc);
 }
 
-}  // namespace
+// Expected level sizes: a candidate from a segment of relation r passes a
+// check of relation c with probability min(1, mean length of c / vertices c
+// reaches); each survivor has (mean length of the next generator) children.
+void project_levels(const HostGraph &data, const Plan &plan, uint64_t nroots, uint64_t g1, double *nodes,
+                    double *cand)
+{
+    const uint32_t q = plan.q;
+    double S = (double)nroots, G = (double)g1;
+    for (uint32_t k = 0; k < q; k++) nodes[k] = cand[k] = 0;
+    if (!q) return;
+    nodes[0] = S;
+    cand[0] = S;
+    for (uint32_t d = 1; d < q; d++) {
+        const DepthPlanDev &dp = plan.dev.depth[d];
+        cand[d] = G;
+        double p = 1;
+        for (uint32_t r = 0; r < dp.nchk; r++) {
+            double m, reach;
+            rel_model(data, dp.chk_rel[r], m, reach);
+            p *= std::min(1.0, m / reach);
+        }
+        S = std::min(G * p, 1e30);
+        nodes[d] = S;
+        double m, reach;
+        rel_model(data, dp.nxt_rel, m, reach);
+        G = std::min(S * m, 1e30);
+    }
+}
 
 Status plan_query(const HostGraph &qg, Plan &out, const HostGraph *by_cost)
 {

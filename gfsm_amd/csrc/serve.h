@@ -40,6 +40,7 @@ struct BatchJob {
     gfsm_match_opts opts{};
     Plan plan;
     uint64_t region_off = 0, region_bytes = 0;
+    double model_bytes = 0;  // projected level bytes before calibration
     double t_admit = 0;
 };
 
@@ -65,6 +66,10 @@ public:
     // Device bytes a query is given: a projection of its level sizes from
     // the data graph's relation statistics, clamped to [min, max] region.
     uint64_t project_bytes(const Plan &plan, uint64_t nroots, uint64_t g1) const;
+    double model_bytes(const Plan &plan, uint64_t nroots, uint64_t g1) const;
+    // Learn the model's error from a finished query (actual / model level bytes).
+    void calibrate(double model, const Result &r, uint32_t q);
+    double calibration() const { return calib_; }
     uint64_t min_region(uint32_t q) const;
     uint64_t max_region() const { return max_region_; }
     void set_max_region(uint64_t b) { max_region_ = b; }
@@ -108,6 +113,8 @@ private:
     uint32_t running_ = 0;
     uint64_t steps_ = 0, since_clear_ = 0;
     uint64_t step_cand_ = 0;             // per-slot step size last written to the device
+    double calib_ = 1, calib_log_ = 0;   // projection calibration (see calibrate)
+    uint64_t calib_n_ = 0;
     uint64_t *step_cand_h_ = nullptr;    // pinned source of that write
     AdmitEntry *admit_h_ = nullptr, *admit_d_ = nullptr;  // admission queue (pinned, device)
     uint32_t *n_admit_h_ = nullptr;

@@ -170,6 +170,14 @@ gfsm_status gfsm_query_roots(const gfsm_graph *data, const gfsm_graph *query, ui
  * (col[query size]).  The reference's order, or with GFSM_ORDER_COST the one
  * chosen from data's relation statistics (no reference counterpart). */
 gfsm_status gfsm_plan_order(const gfsm_graph *data, const gfsm_graph *query, uint32_t flags, uint32_t *col);
+/* Projected size of every depth (query-size entries each, depth order): the
+ * expected nodes kept and candidates tested, from data's relation statistics
+ * and the plan that `opts` selects (flags, root window).  Host only, no device.
+ * The batch engine sizes query regions from it, scaled by a calibration learnt
+ * from finished queries.  No reference counterpart (the reference sizes every
+ * level at SOLN_LIMIT, HnGPU.cu:849-856). */
+gfsm_status gfsm_query_projection(const gfsm_graph *data, const gfsm_graph *query, const gfsm_match_opts *opts,
+                                  double *nodes, double *cand);
 
 /* A root window for bounded work (no counterpart): the largest set of the
  * highest-id root candidates whose depth-1 candidates total at most max_first

@@ -310,3 +310,31 @@ def test_read_fs_reads_the_oracle_layout_and_rejects_damage(tmp_path):
     O.fs_write(empty, np.zeros((0, 4), np.uint32))
     assert G.read_fs(empty).shape == (0, 4)
     assert G.format_rows(rows, limit=2) == "0->3 1->1 2->4\n0->1 1->5 2->9\n... 1 more"
+
+
+@pytest.mark.parametrize("mode,nvl", [(G.MODE_REF, 1), (G.MODE_VLABEL, 3)])
+def test_level_projection_tracks_the_oracle_on_random_graphs(mode, nvl):
+    """The projector's expected level sizes (relation statistics, checks as
+    independent events) against the oracle's exact level sizes on ER graphs:
+    depth 0 is the root candidate count, depth 1 is exact without checks, and
+    deeper levels stay within 1.6x (counting vertices without a segment of a
+    relation as length 0 matters: without it labelled levels drift 2-5x)."""
+    data = G.Graph.er(3000, 8.0, nvlabels=nvl, nelabels=2, seed=5, mode=mode)
+    for s in range(3, 7):
+        for kind in (G.WALK_DENSE, G.WALK_SPARSE):
+            q = G.Graph.walk_query(data, s, kind=kind, seed=100 + s * 3 + kind)
+            nodes, cand = data.projection(q)
+            assert nodes[0] == len(data.roots(q))
+            lv = np.array(oracle_levels(data, q, mode), float)
+            for k in range(1, s):
+                if lv[k] >= 200:
+                    assert 1 / 1.6 <= nodes[k] / lv[k] <= 1.6, (s, kind, k, nodes.tolist(), lv.tolist())
+    q = G.Graph.walk_query(data, 4, seed=7)
+    roots = data.roots(q)
+    lo, hi = int(roots[10]), int(roots[20])
+    nodes, _ = data.projection(q, root_lo=lo, root_hi=hi)
+    assert nodes[0] == 10
+
+
+def oracle_levels(data, q, mode):
+    return O.solve(to_oracle(data), to_oracle(q), mode=mode, rows=False).level

@@ -248,6 +248,36 @@ Status plan_for(const HostGraph &data, const HostGraph &query, const gfsm_match_
     return plan_query(query, plan, &data);
 }
 
+const uint2 *dense_segments(const HostGraph &data, DevGraph &dg, uint32_t rel, uint64_t lookups, hipStream_t s)
+{
+    std::lock_guard<std::mutex> lk(dg.dense_mu);
+    auto it = dg.dense.find(rel);
+    if (it != dg.dense.end()) return it->second;
+    const uint64_t bytes = (uint64_t)data.V * sizeof(uint2);
+    if (lookups < data.V / 8 || dg.dense_bytes + bytes > kDenseMaxBytes) return nullptr;
+    uint2 *t = nullptr;
+    if (hipMalloc(&t, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    uint64_t lo = 0, hi = 0;
+    const int64_t k = data.rel_index(rel);
+    if (k >= 0) {
+        lo = data.rel_off[k];
+        hi = data.rel_off[k + 1];
+    }
+    // Built to completion before it is shared: other streams may read it next.
+    if (launch_dense_seg(dg.rel_vtx + lo, dg.rel_beg + lo, dg.rel_cnt + lo, hi - lo, data.V, t, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipFree(t);
+        return nullptr;
+    }
+    dg.dense[rel] = t;
+    dg.dense_bytes += bytes;
+    return t;
+}
+
 void root_range(const HostGraph &data, const Plan &plan, uint32_t root_lo, uint32_t root_hi, uint64_t &lo,
                 uint64_t &hi)
 {
@@ -314,8 +344,6 @@ namespace {
 // batch engine (whose steps cost nothing per query).
 constexpr uint64_t kCapStartDiv = 8;
 
-// Device bytes of dense segment tables one resident graph may hold.
-constexpr uint64_t kDenseMaxBytes = 4ull << 30;
 
 // Steps of at least this many tiles get their frontier nodes from k_tile_index.
 constexpr uint64_t kTileIndexMin = 64;
@@ -378,37 +406,6 @@ struct Runner {
         const uint64_t by_ws = ctx.max_tiles * kTile;  // enough tiles at any tile size
         const uint64_t by_val = kValueMask / std::max<uint64_t>(1, data.max_degree);
         return std::max<uint64_t>(kTile * 8, std::min(by_ws, by_val));
-    }
-
-    // Dense segment table of relation `rel` for a launch testing `ncand`
-    // candidates: cached on the mirror; built when the launch makes at least
-    // V/8 lookups, within kDenseMaxBytes per mirror.  Null: hash lookups.
-    const uint2 *dense_table(uint32_t rel, uint64_t ncand)
-    {
-        DevGraph &m = const_cast<DevGraph &>(dg);
-        auto it = m.dense.find(rel);
-        if (it != m.dense.end()) return it->second;
-        const uint64_t bytes = (uint64_t)data.V * sizeof(uint2);
-        if (ncand < data.V / 8 || m.dense_bytes + bytes > kDenseMaxBytes) return nullptr;
-        uint2 *t = nullptr;
-        if (hipMalloc(&t, bytes) != hipSuccess) {
-            (void)hipGetLastError();
-            return nullptr;
-        }
-        uint64_t lo = 0, hi = 0;
-        const int64_t k = data.rel_index(rel);
-        if (k >= 0) {
-            lo = data.rel_off[k];
-            hi = data.rel_off[k + 1];
-        }
-        if (launch_dense_seg(dg.rel_vtx + lo, dg.rel_beg + lo, dg.rel_cnt + lo, hi - lo, data.V, t,
-                             ctx.lane.stream) != hipSuccess) {
-            (void)hipFree(t);
-            return nullptr;
-        }
-        m.dense[rel] = t;
-        m.dense_bytes += bytes;
-        return t;
     }
 
     uint32_t grid_for(uint64_t ntiles) const
@@ -588,9 +585,12 @@ struct Runner {
             std::memcpy(a.dcol, plan.dev.dcol, sizeof(a.dcol));
             StepIO &io = a.io;
             io.need_nxt = need_nxt(kind, d);
-            if (kind == EXPAND_MID) io.nxt_seg = dense_table(plan.dev.depth[d].nxt_rel, e - c);
+            if (kind == EXPAND_MID)
+                a.dp.nxt_seg = dense_segments(data, const_cast<DevGraph &>(dg), plan.dev.depth[d].nxt_rel, e - c,
+                                              ctx.lane.stream);
             for (uint32_t r = 0; r < std::min<uint32_t>(plan.dev.depth[d].nchk, kDenseChk); r++)
-                io.chk_seg[r] = dense_table(plan.dev.depth[d].chk_rel[r], e - c);
+                a.dp.chk_seg[r] = dense_segments(data, const_cast<DevGraph &>(dg), plan.dev.depth[d].chk_rel[r],
+                                                 e - c, ctx.lane.stream);
             io.d = d;
             io.q = q;
             io.kind = kind;

@@ -25,9 +25,16 @@ struct DevGraph {
     size_t bytes = 0;
     // Dense per-relation segment tables ({begin, end} for every vertex), built
     // on first use by the single-query path and kept while the mirror lives.
+    std::mutex dense_mu;
     std::unordered_map<uint32_t, uint2 *> dense;
     uint64_t dense_bytes = 0;
 };
+
+// Dense segment table of relation `rel` on a resident graph: cached on the
+// mirror; built (on stream `s`, waited for) when a step makes at least V/8
+// lookups of it, within kDenseMaxBytes per mirror.  Null: use the hash table.
+const uint2 *dense_segments(const HostGraph &data, DevGraph &dg, uint32_t rel, uint64_t lookups, hipStream_t s);
+constexpr uint64_t kDenseMaxBytes = 4ull << 30;
 
 // A stream with its own chained-scan workspace and result block.  Kernels of
 // one lane run in stream order, so the workspace is never shared by two

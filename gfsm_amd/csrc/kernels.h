@@ -89,13 +89,8 @@ struct StepIO {
     uint32_t pad;
     const uint64_t *tile_node; // [2*ntiles] first and last frontier node of each tile (k_tile_index),
                                // or null: each tile searches in_off itself
-    // Dense segment tables ([V] {begin, end} of every vertex's segment of one
-    // relation) for the depth's next relation and its first kDenseChk checked
-    // relations; null entries use the segment hash table.
-    const uint2 *nxt_seg;
-    const uint2 *chk_seg[4];
 };
-constexpr uint32_t kDenseChk = 4;
+constexpr uint32_t kDenseChk = 4;  // checked relations per depth with a dense-table slot (DepthPlanDev)
 
 // Root step: the candidates of depth 0 (a slice of the relation index).
 struct RootIO {

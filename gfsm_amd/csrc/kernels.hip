@@ -577,7 +577,7 @@ DEV void expand_tile(const DevGraphView &g, const DP &dp, CP col, CP dcol,
         for (uint32_t u0 = tid; u0 < total; u0 += T * L) {
             uint32_t x[L], rel[L], dst[L];
             bool act[L];
-            const uint2 *tab[L];
+            const void *tab[L];
 #pragma unroll
             for (uint32_t k = 0; k < L; k++) {
                 const uint32_t u = u0 + k * T;
@@ -593,7 +593,7 @@ DEV void expand_tile(const DevGraphView &g, const DP &dp, CP col, CP dcol,
                     const uint32_t col = (r < nchk) ? d - dp.chk_dist[r] : d + 1 - dp.nxt_dist;
                     rel[k] = (r < nchk) ? dp.chk_rel[r] : dp.nxt_rel;
                     dst[k] = (r < nchk) ? r * nf + j : nchk * nf + j;  // s_chk[r][j] or s_nxt[j]
-                    tab[k] = (r < nchk) ? (r < kDenseChk ? io.chk_seg[r] : nullptr) : io.nxt_seg;
+                    tab[k] = (r < nchk) ? (r < kDenseChk ? dp.chk_seg[r] : nullptr) : dp.nxt_seg;
                     if (act[k]) x[k] = in_rows[(uint64_t)j * d + col];
                 }
             }
@@ -671,7 +671,7 @@ DEV void expand_tile(const DevGraphView &g, const DP &dp, CP col, CP dcol,
                 uint32_t x[I];
 #pragma unroll
                 for (uint32_t k = 0; k < I; k++) x[k] = dist ? rows[jj[k] * d + (d - dist)] : w[k];
-                const uint2 *ct = r < kDenseChk ? io.chk_seg[r] : nullptr;
+                const void *ct = r < kDenseChk ? dp.chk_seg[r] : nullptr;
                 if (ct) {
                     const auto tab = gp(reinterpret_cast<const uint64_t *>(ct));
 #pragma unroll
@@ -692,9 +692,9 @@ DEV void expand_tile(const DevGraphView &g, const DP &dp, CP col, CP dcol,
             if (need_nxt && stage_meta) {
 #pragma unroll
                 for (uint32_t k = 0; k < I; k++) ns[k] = s_nxt[jj[k]];
-            } else if (io.nxt_seg) {
+            } else if (dp.nxt_seg) {
                 // one read of the relation's dense table ({begin, end} words)
-                const auto tab = gp(reinterpret_cast<const uint64_t *>(io.nxt_seg));
+                const auto tab = gp(reinterpret_cast<const uint64_t *>(dp.nxt_seg));
 #pragma unroll
                 for (uint32_t k = 0; k < I; k++) {
                     const uint32_t x = need_nxt ? rows[jj[k] * d + (d + 1 - dp.nxt_dist)] : w[k];

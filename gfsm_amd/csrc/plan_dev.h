@@ -22,6 +22,11 @@ struct DepthPlanDev {
     uint32_t nxt_rel;
     uint32_t chk_dist[GFSM_MAXCHK];
     uint32_t chk_rel[GFSM_MAXCHK];
+    // Dense segment tables ({begin, end} per data vertex, see DevGraph) of
+    // nxt_rel and of the first four chk_rel, filled in per run where one
+    // exists; null: the segment hash table.
+    const void *nxt_seg;
+    const void *chk_seg[4];
 };
 
 struct QueryPlanDev {

@@ -28,16 +28,20 @@ uint64_t mix64(uint64_t z)
     return z ^ (z >> 31);
 }
 
+// Row hash of GFSM_DIGEST: per seed, the finalised sum over columns of
+// mix64(seed ^ (row[c] | c << 32)).  A sum inside each row lets a kernel add
+// a node's placed columns once for all its candidates; the final mix keeps
+// rows apart (sums over rows of unmixed column terms would not).
 void row_hash(const uint32_t *row, uint32_t width, uint64_t h[2])
 {
-    uint64_t a = 0x9E3779B97F4A7C15ull, b = 0xD1B54A32D192ED03ull;
+    uint64_t a = 0, b = 0;
     for (uint32_t c = 0; c < width; c++) {
-        uint64_t x = (uint64_t)row[c] | ((uint64_t)c << 32);
-        a = mix64(a ^ x);
-        b = mix64(b ^ x);
+        const uint64_t x = (uint64_t)row[c] | ((uint64_t)c << 32);
+        a += mix64(0x9E3779B97F4A7C15ull ^ x);
+        b += mix64(0xD1B54A32D192ED03ull ^ x);
     }
-    h[0] = a;
-    h[1] = b;
+    h[0] = mix64(a);
+    h[1] = mix64(b);
 }
 
 bool relation_of(uint32_t mode, uint32_t lu, uint32_t lv, uint32_t l, uint32_t &rel)

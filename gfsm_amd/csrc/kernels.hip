@@ -554,6 +554,12 @@ DEV void expand_tile(const DevGraphView &g, const DP &dp, CP col, CP dcol,
     uint32_t *s_rows = s_pool;                                                   // [nf][d]
     uint2 *s_chk = reinterpret_cast<uint2 *>(s_pool + (staged ? row_words : 0));  // [nchk][nf]
     uint2 *s_nxt = s_chk + nchk * nf;                                            // [nf]
+    // Digest of a counting last depth: each node's placed columns contribute
+    // the same terms to all its candidates' row hashes, so their sums are
+    // formed once per node (two u64 per node, after the staged data).
+    const uint32_t pool_used = (staged ? row_words : 0u) + (stage_meta ? meta_words : 0u);
+    const bool part_ok = KIND == EXPAND_LAST_COUNT && io.digest && pool_used + 4 * nf <= pool_words;
+    uint64_t *s_part = reinterpret_cast<uint64_t *>(s_pool + pool_used);         // [nf][2]
     const auto in_rows = gp(io.in_vec) + ilo * d;       // the tile's node rows, contiguous
 
     if (staged)
@@ -565,6 +571,18 @@ DEV void expand_tile(const DevGraphView &g, const DP &dp, CP col, CP dcol,
         const uint32_t first = (o > c0) ? (uint32_t)(o - c0) : 0u;
         atomicOr((unsigned long long *)&s_bits[first >> 6], 1ull << (first & 63));
         s_seg[j] = in_gbeg[i] + (uint32_t)(c0 - o);
+        if (part_ok) {
+            uint64_t p0 = 0, p1 = 0;
+            for (uint32_t c = 0; c < io.q; c++) {
+                const uint32_t dc = dcol[c];
+                if (dc == d) continue;
+                const uint64_t xv = (uint64_t)in_rows[(uint64_t)j * d + dc] | ((uint64_t)c << 32);
+                p0 += mix64(0x9E3779B97F4A7C15ull ^ xv);
+                p1 += mix64(0xD1B54A32D192ED03ull ^ xv);
+            }
+            s_part[2 * j] = p0;
+            s_part[2 * j + 1] = p1;
+        }
     }
     // Staged segments: every (node, lookup) pair of the tile -- the node's
     // checked ancestors and its next generator -- striped over the threads
@@ -811,20 +829,29 @@ DEV void expand_tile(const DevGraphView &g, const DP &dp, CP col, CP dcol,
 #pragma unroll
             for (uint32_t k = 0; k < I; k++) acc_cnt += keep[k] ? 1u : 0u;
         }
-        if (KIND != EXPAND_MID && !BARE && io.digest) {
+        if (KIND != EXPAND_MID && !BARE && io.digest && part_ok) {
+            const uint64_t cw = (uint64_t)col[d] << 32;
 #pragma unroll
             for (uint32_t k = 0; k < I; k++) {
                 if (!keep[k]) continue;
-                uint64_t x0 = 0x9E3779B97F4A7C15ull, x1 = 0xD1B54A32D192ED03ull;
+                const uint64_t xv = (uint64_t)w[k] | cw;
+                acc_h0 += mix64(s_part[2 * jj[k]] + mix64(0x9E3779B97F4A7C15ull ^ xv));
+                acc_h1 += mix64(s_part[2 * jj[k] + 1] + mix64(0xD1B54A32D192ED03ull ^ xv));
+            }
+        } else if (KIND != EXPAND_MID && !BARE && io.digest) {
+#pragma unroll
+            for (uint32_t k = 0; k < I; k++) {
+                if (!keep[k]) continue;
+                uint64_t x0 = 0, x1 = 0;
                 for (uint32_t c = 0; c < io.q; c++) {
                     const uint32_t dc = dcol[c];
                     const uint32_t val = (dc == d) ? w[k] : rows[jj[k] * d + dc];
                     const uint64_t xv = (uint64_t)val | ((uint64_t)c << 32);
-                    x0 = mix64(x0 ^ xv);
-                    x1 = mix64(x1 ^ xv);
+                    x0 += mix64(0x9E3779B97F4A7C15ull ^ xv);
+                    x1 += mix64(0xD1B54A32D192ED03ull ^ xv);
                 }
-                acc_h0 += x0;
-                acc_h1 += x1;
+                acc_h0 += mix64(x0);
+                acc_h1 += mix64(x1);
             }
         }
     };

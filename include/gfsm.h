@@ -205,7 +205,8 @@ gfsm_status gfsm_match_write_fs(gfsm_ctx *ctx, const gfsm_graph *data, const gfs
                                 const gfsm_match_opts *opts, const char *path, gfsm_result **r);
 void        gfsm_result_free(gfsm_result *r);
 
-/* Row digest used by GFSM_DIGEST: h = sum over rows of two seeded hashes of
+/* Row digest used by GFSM_DIGEST (per seed: mix(sum over columns c of
+ * mix(seed ^ (row[c] | c << 32))), mix = the SplitMix64 finaliser): h = sum over rows of two seeded hashes of
  * the row (column order = original query id). */
 void        gfsm_row_hash(const uint32_t *row, uint32_t width, uint64_t h[2]);
 

@@ -518,8 +518,8 @@ void orc_plan_free(orc_plan *p)
 /* ------------------------------------------------------------------------ */
 
 /* Row digest: a multiset hash, summed over rows mod 2^64 with two seeds.
- * acc = seed; for each column c: acc = mix(acc ^ (row[c] | c << 32)), where
- * mix is the SplitMix64 finaliser. */
+ * Per seed: h = mix(sum over columns c of mix(seed ^ (row[c] | c << 32))),
+ * where mix is the SplitMix64 finaliser. */
 static inline uint64_t mix64(uint64_t z)
 {
     z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
@@ -529,14 +529,14 @@ static inline uint64_t mix64(uint64_t z)
 
 void orc_row_hash(const uint32_t *row, uint32_t width, uint64_t h[2])
 {
-    uint64_t a = 0x9E3779B97F4A7C15ull, b = 0xD1B54A32D192ED03ull;
+    uint64_t a = 0, b = 0;
     for (uint32_t c = 0; c < width; c++) {
         uint64_t x = (uint64_t)row[c] | ((uint64_t)c << 32);
-        a = mix64(a ^ x);
-        b = mix64(b ^ x);
+        a += mix64(0x9E3779B97F4A7C15ull ^ x);
+        b += mix64(0xD1B54A32D192ED03ull ^ x);
     }
-    h[0] = a;
-    h[1] = b;
+    h[0] = mix64(a);
+    h[1] = mix64(b);
 }
 
 /* reqExists (HnGPU.cu:34-52): is there an entry (rel, vertex) in the row at

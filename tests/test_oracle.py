@@ -324,11 +324,12 @@ def mix64(z):
 
 
 def py_row_hash(row):
-    a, b = 0x9E3779B97F4A7C15, 0xD1B54A32D192ED03
+    a = b = 0
     for c, v in enumerate(row):
         x = int(v) | (c << 32)
-        a, b = mix64(a ^ x), mix64(b ^ x)
-    return a, b
+        a = (a + mix64(0x9E3779B97F4A7C15 ^ x)) % (1 << 64)
+        b = (b + mix64(0xD1B54A32D192ED03 ^ x)) % (1 << 64)
+    return mix64(a), mix64(b)
 
 
 def test_row_hash_and_digest():

@@ -145,8 +145,9 @@ hipError_t launch_dense_seg(const uint32_t *vtx, const uint32_t *beg, const uint
                             uint2 *out, hipStream_t s);
 // Frontier nodes of every tile of an expand step, searched ahead of it with
 // the whole GPU: out[2t] = the node holding candidate g_begin + t * tc,
-// out[2t+1] = the node holding the tile's last candidate.  A tile then reads
-// two words instead of walking the offsets with dependent loads.
+// out[2t+1] = the node holding the tile's last candidate (offsets must be
+// strictly increasing, as every level's are).  A tile then reads two words
+// instead of walking the offsets with dependent loads.
 hipError_t launch_tile_index(const uint64_t *off, uint64_t S, uint64_t g_begin, uint64_t g_end, uint64_t tc,
                              uint64_t ntiles, uint64_t *out, hipStream_t s);
 

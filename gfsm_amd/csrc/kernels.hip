@@ -1282,17 +1282,24 @@ __global__ void k_dense_seg(const uint32_t *vtx, const uint32_t *beg, const uint
 
 // One half-wave per search key: key 2t is tile t's first candidate, key 2t+1
 // its last.
+// One search per tile boundary: key t < ntiles is tile t's first candidate,
+// key ntiles the step's last.  Every frontier node has at least one child
+// (offsets strictly increase), so the candidate before a boundary belongs to
+// the boundary's node when that node starts earlier, else to the node before.
 __global__ __launch_bounds__(kTile) void k_tile_index(const uint64_t *off, uint64_t S, uint64_t g_begin,
                                                       uint64_t g_end, uint64_t tc, uint64_t ntiles, uint64_t *out)
 {
-    const uint64_t key_id = ((uint64_t)blockIdx.x * kTile + threadIdx.x) >> 5;
-    if (key_id >= 2 * ntiles) return;  // whole half-waves leave together (32 | kTile)
-    const uint64_t t = key_id >> 1;
-    const uint64_t c0 = g_begin + t * tc;
-    const uint64_t c1 = (c0 + tc < g_end) ? c0 + tc : g_end;
-    const uint64_t key = (key_id & 1) ? c1 - 1 : c0;
+    const uint64_t t = ((uint64_t)blockIdx.x * kTile + threadIdx.x) >> 5;
+    if (t > ntiles) return;  // whole half-waves leave together (32 | kTile)
+    const uint64_t key = (t < ntiles) ? g_begin + t * tc : g_end - 1;
     const uint64_t i = frontier_search(off, S, key, threadIdx.x & 31);
-    if ((threadIdx.x & 31) == 0) out[key_id] = i;
+    if ((threadIdx.x & 31) != 0) return;
+    if (t == ntiles) {
+        out[2 * t - 1] = i;  // the last tile's last node
+        return;
+    }
+    out[2 * t] = i;
+    if (t > 0) out[2 * t - 1] = (off[i] < key || i == 0) ? i : i - 1;
 }
 
 __global__ void k_seg_fill(uint4 *slots, uint32_t mask)
@@ -1437,7 +1444,7 @@ hipError_t launch_tile_index(const uint64_t *off, uint64_t S, uint64_t g_begin, 
 {
     if (ntiles == 0 || S == 0 || g_end <= g_begin) return hipErrorInvalidValue;
     const uint64_t keys_per_wg = kTile / 32;
-    const uint64_t grid = (2 * ntiles + keys_per_wg - 1) / keys_per_wg;
+    const uint64_t grid = (ntiles + keys_per_wg) / keys_per_wg;  // ntiles + 1 keys
     hipLaunchKernelGGL(k_tile_index, dim3((uint32_t)grid), dim3(kTile), 0, s, off, S, g_begin, g_end, tc, ntiles, out);
     return hipGetLastError();
 }

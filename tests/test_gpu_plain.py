@@ -111,6 +111,23 @@ def test_chunked_equals_unchunked(ctx):
     assert np.array_equal(rows_full.rows, rows_small.rows)
 
 
+def test_chunked_steps_with_tile_index(ctx):
+    """Chunked expansion whose chunks start past candidate 0 and still span
+    hundreds of tiles, so the tile index runs on offset ranges: counts and
+    digests equal the unchunked match and the oracle (34.7M embeddings)."""
+    data = G.Graph.er(20000, 12.0, nvlabels=1, nelabels=1, seed=81, mode=G.MODE_REF)
+    query = G.Graph.walk_query(data, 4, kind=G.WALK_SPARSE, seed=8104)
+    ref = oracle_solve(data, query, G.MODE_REF, rows=False)
+    full = ctx.match(data, query, digest=True)
+    assert (full.count, full.digest) == (ref.count, ref.digest)
+    assert full.stats["chunks"] == 0
+    for budget in (32 << 20, 48 << 20):
+        m = ctx.match(data, query, digest=True, mem_budget=budget)
+        assert (m.count, m.digest) == (ref.count, ref.digest), budget
+        assert m.stats["chunks"] > 0
+        assert ctx.match(data, query, mem_budget=budget).count == ref.count  # counting last depth
+
+
 def test_root_range_partition(ctx):
     data = G.Graph.er(1500, 7.0, nvlabels=2, seed=77, mode=G.MODE_VLABEL)
     query = G.Graph.walk_query(data, 5, kind=G.WALK_SPARSE, seed=78)

@@ -50,10 +50,7 @@ Status Lane::init(uint64_t max_tiles, bool tile_index)
     HIP_TRY(hipMemset(tot_d, 0, sizeof(Totals)));
     HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&tot_h), sizeof(Totals), hipHostMallocDefault));
     std::memset(tot_h, 0, sizeof(Totals));
-    for (int i = 0; i < 2; i++) {
-        HIP_TRY(hipEventCreate(&ev[i]));
-        HIP_TRY(hipEventCreate(&kev[i]));
-    }
+    for (int i = 0; i < 2; i++) HIP_TRY(hipEventCreate(&ev[i]));
     return Status::ok();
 }
 
@@ -77,10 +74,9 @@ void Lane::destroy()
     tile_node = nullptr;
     (void)hipFree(tot_d);
     if (tot_h) (void)hipHostFree(tot_h);
-    for (int i = 0; i < 2; i++) {
+    for (int i = 0; i < 2; i++)
         if (ev[i]) (void)hipEventDestroy(ev[i]);
-        if (kev[i]) (void)hipEventDestroy(kev[i]);
-    }
+    for (hipEvent_t e : kev) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
     *this = Lane{};
 }
@@ -481,22 +477,46 @@ struct Runner {
         return Status::ok();
     }
 
-    void kbegin()
+    // Per-launch timing (GFSM_PROFILE): an event pair around each matcher
+    // launch, read back in batches, so timing adds no host wait between
+    // launches beyond those the depth loop makes anyway.
+    static constexpr uint32_t kProfPairs = 512;
+    uint32_t kn = 0;                     // pairs recorded and not yet read
+    bool kexp[kProfPairs] = {};          // pair i timed an expand launch
+    Status kbegin()
     {
-        if (prof) (void)hipEventRecord(ctx.lane.kev[0], ctx.lane.stream);
+        if (!prof) return Status::ok();
+        std::vector<hipEvent_t> &ev = ctx.lane.kev;
+        while (ev.size() < 2ull * (kn + 1)) {
+            hipEvent_t e = nullptr;
+            HIP_TRY(hipEventCreate(&e));
+            ev.push_back(e);
+        }
+        HIP_TRY(hipEventRecord(ev[2 * kn], ctx.lane.stream));
+        return Status::ok();
     }
     Status kend(bool expand)
     {
         if (!prof) return Status::ok();
-        HIP_TRY(hipEventRecord(ctx.lane.kev[1], ctx.lane.stream));
-        HIP_TRY(hipEventSynchronize(ctx.lane.kev[1]));
-        float ms = 0;
-        HIP_TRY(hipEventElapsedTime(&ms, ctx.lane.kev[0], ctx.lane.kev[1]));
-        res.stats.kernel_ms += ms;
-        if (expand) {
-            res.stats.expand_ms += ms;
-            res.stats.expand_launches++;
+        HIP_TRY(hipEventRecord(ctx.lane.kev[2 * kn + 1], ctx.lane.stream));
+        kexp[kn++] = expand;
+        return kn == kProfPairs ? prof_flush() : Status::ok();
+    }
+    Status prof_flush()
+    {
+        if (kn == 0) return Status::ok();
+        const std::vector<hipEvent_t> &ev = ctx.lane.kev;
+        HIP_TRY(hipEventSynchronize(ev[2 * kn - 1]));
+        for (uint32_t i = 0; i < kn; i++) {
+            float ms = 0;
+            HIP_TRY(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
+            res.stats.kernel_ms += ms;
+            if (kexp[i]) {
+                res.stats.expand_ms += ms;
+                res.stats.expand_launches++;
+            }
         }
+        kn = 0;
         return Status::ok();
     }
 
@@ -529,7 +549,10 @@ struct Runner {
             if (a.io.ntiles > ctx.max_tiles || e > data.rel_vtx.size())
                 return Status::err(GFSM_ERR_ARG, "internal: root launch shape check failed");
             HIP_TRY(ctx.lane.before_launch());
-            kbegin();
+            {
+                Status ks = kbegin();
+                if (!ks.good()) return ks;
+            }
             HIP_TRY(launch_root(a, grid_for(a.io.ntiles), ctx.lane.stream));
             res.stats.launches++;
             Status st = kend(false);
@@ -629,7 +652,10 @@ struct Runner {
                 io.tile_node = ctx.lane.tile_node;
             }
             HIP_TRY(ctx.lane.before_launch());
-            kbegin();
+            {
+                Status ks = kbegin();
+                if (!ks.good()) return ks;
+            }
             HIP_TRY(launch_expand(a, (uint32_t)std::min<uint64_t>(io.ntiles, (uint64_t)grid_cap * grid_mul),
                                   ctx.lane.stream));
             res.stats.launches++;
@@ -794,6 +820,8 @@ Status match_plain(Ctx &ctx, const HostGraph &data, const HostGraph &query, cons
     }
     HIP_TRY(hipEventRecord(L.ev[1], L.stream));
     st = run.sync_totals();
+    if (!st.good()) return st;
+    st = run.prof_flush();
     if (!st.good()) return st;
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, L.ev[0], L.ev[1]));

@@ -45,7 +45,7 @@ struct Lane {
     Totals *tot_d = nullptr;
     Totals *tot_h = nullptr;  // pinned
     hipEvent_t ev[2] = {nullptr, nullptr};
-    hipEvent_t kev[2] = {nullptr, nullptr};
+    std::vector<hipEvent_t> kev;  // begin/end pairs of timed launches (grown on demand)
     uint64_t launches = 0;  // chained-scan launches issued on this lane
     uint64_t *tile_node = nullptr;  // [2*max_tiles] per-tile frontier nodes (single-query lane only)
     Status init(uint64_t max_tiles, bool tile_index = false);

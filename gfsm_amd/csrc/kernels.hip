@@ -588,7 +588,7 @@ DEV void expand_tile(const DevGraphView &g, const DP &dp, CP col, CP dcol,
     // checked ancestors and its next generator -- striped over the threads
     // and resolved kNodeLk at a time in lockstep, so a node's lookups do not
     // run one after another.
-    if (stage_meta) {
+    if (!BARE && stage_meta) {  // the lean kind has no lookups to stage
         constexpr uint32_t L = kNodeLk;
         const uint32_t nl = nchk + need_nxt;
         const uint32_t total = nf * nl;

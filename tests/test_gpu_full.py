@@ -2,7 +2,8 @@
 hold at any size: the oracle on the windows it finishes in seconds, counts and
 digests that add over a split of the root window, chunked == unchunked, the
 lean counting kernel == the digest kernel, closed forms on the full
-1400 x 1400 grid, and batched == single query on the scale-22 graph."""
+1400 x 1400 grid, batched == single query on the scale-22 graph, and served
+== alone on the resident scale-23 graph."""
 import os
 
 import numpy as np
@@ -118,3 +119,26 @@ def test_config4_batched_equals_single(ctx):
         one = ctx.match(data, q, root_lo=lo, root_hi=hi, max_cand=cap, max_depth_cand=dcap, allow_partial=True,
                         digest=True)
         assert (m.count, m.digest, m.status) == (one.count, one.digest, one.status), i
+
+
+def test_config5_served_equals_alone(ctx):
+    """R-MAT scale 23, edge factor 16, resident: the first 512 queries of the
+    served stream (4-24 vertices, deterministic caps) through the server with
+    a 200 GB admission budget equal each query run alone."""
+    data = W.config5_data()
+    qs = W.config5_queries(data, 512)
+    wins = W.config5_windows(data, qs)
+    cap, last = W.CONFIG5_CAP, W.CONFIG5_DEPTH_CAP
+    srv = G.Server(ctx, data, mem_budget=200 * 10**9, max_inflight=256, digest=True)
+    try:
+        tk = [srv.submit(q, root_lo=lo, root_hi=hi, max_cand=cap, max_depth_cand=last) for q, (lo, hi) in zip(qs, wins)]
+        got = [srv.wait(x) for x in tk]
+    finally:
+        srv.stop()
+    capped = 0
+    for i, (q, (lo, hi), m) in enumerate(zip(qs, wins, got)):
+        one = ctx.match(data, q, root_lo=lo, root_hi=hi, max_cand=cap, max_depth_cand=last, allow_partial=True,
+                        digest=True)
+        assert (m.count, m.digest, m.status) == (one.count, one.digest, one.status), i
+        capped += one.status != 0
+    assert 0 < capped < len(qs)  # both capped and finished queries are compared

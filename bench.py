@@ -249,8 +249,8 @@ def main():
         calibrate_copy(args.calibrate_copy)
         return
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              f"(the multi-GPU design shards roots, see the design notes)", file=sys.stderr)
+        print(f"bench.py: the matcher runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(several GPUs would each take a share of the root candidates; not implemented)", file=sys.stderr)
         sys.exit(2)
     import gfsm_amd as G
     import gfsm_amd.workloads as W

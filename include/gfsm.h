@@ -58,7 +58,7 @@ typedef int32_t gfsm_status;
 #define GFSM_ERR_ARG 2            /* bad argument / out-of-domain input                      */
 #define GFSM_ERR_NOMEM 3          /* device or host allocation failed, or budget too small   */
 #define GFSM_ERR_DEVICE 4         /* HIP runtime error                                       */
-#define GFSM_ERR_CAP 5            /* more rows than the caller's max_rows                    */
+#define GFSM_ERR_CAP 5            /* a cap was reached (max_rows, max_cand, max_depth_cand)  */
 #define GFSM_ERR_PENDING 6        /* server: result not ready yet                            */
 #define GFSM_ERR_MISSING_FILE 232 /* HnHelper.cuh:16 */
 #define GFSM_ERR_INVALID_DATA 233 /* HnHelper.cuh:17 */
@@ -109,7 +109,9 @@ typedef struct gfsm_stats {
     uint64_t level[GFSM_MAX_Q]; /* nodes kept per depth (last depth: embeddings)             */
     uint64_t cand[GFSM_MAX_Q];  /* candidates tested per depth (depth 0: root candidates)     */
     uint32_t checks[GFSM_MAX_Q];/* membership probes per candidate at each depth             */
-    uint64_t alg_bytes;         /* algorithmic HBM bytes of the path (see design notes)      */
+    uint64_t alg_bytes;         /* algorithmic HBM bytes: 4 per root candidate; per depth k, */
+                                /* 8 per frontier node, 4 per candidate and per probe, 8 per */
+                                /* kept inner node; rows mode adds 4 q per embedding         */
     double device_ms;           /* first matcher kernel -> result ready (HIP events)         */
     double kernel_ms;           /* sum of matcher kernel durations (GFSM_PROFILE only)       */
     double expand_ms;           /* sum of expand-kernel durations (GFSM_PROFILE only)        */

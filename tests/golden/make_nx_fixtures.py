@@ -1,5 +1,5 @@
-"""Write brute-force golden vectors with networkx (run in the development
-container only; the GPU box has no networkx).
+"""Write brute-force golden vectors with networkx (run where networkx is
+installed; the tests only read the committed fixtures).
 
 For small seeded graphs and queries, networkx's VF2
 ``DiGraphMatcher(G, Q, node_match, edge_match).subgraph_monomorphisms_iter()``

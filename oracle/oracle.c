@@ -357,10 +357,20 @@ int orc_ccsr_packed(const orc_ccsr *c, uint32_t *out)
     return ORC_OK;
 }
 
-/* hnWrite's id recovery (HnGPU.cu:684-691). */
+/* Length of the row at offset off.  A vertex with no out-entries has no row:
+ * the layout points it one past the last row (offset == size), where the
+ * reference reads beyond its segment array; here that row is empty. */
+static inline uint32_t row_len(const orc_ccsr *c, uint64_t off)
+{
+    return off < c->size ? c->tgt[off] : 0u;
+}
+
+/* hnWrite's id recovery (HnGPU.cu:684-691).  Vertices without a row share
+ * one offset, so their ids cannot be told apart: UINT32_MAX. */
 uint32_t orc_offset_to_id(const orc_ccsr *c, uint32_t offset)
 {
-    uint32_t deg = c->tgt[offset];
+    uint32_t deg = row_len(c, offset);
+    if (deg == 0) return 0xFFFFFFFFu;
     uint64_t rank = c->stagNorm[deg - 1] + (offset - c->stagOffset[deg - 1]) / ((uint64_t)deg + 1);
     return c->invVerticeLocs[rank];
 }
@@ -400,7 +410,7 @@ static int gen_requirements(const orc_ccsr *g, uint32_t *requirements, uint32_t 
     *chosen = relativeDepth;
 
     uint64_t rowOff = localVerticeLocs[relativeDepth];
-    uint32_t deg = g->tgt[rowOff];
+    uint32_t deg = row_len(g, rowOff);
     uint32_t reqOffset = depth * maxValency * 2;
     req_rel *reqRels = (req_rel *)calloc(maxValency ? maxValency : 1, sizeof(req_rel));
     uint32_t numOfReqs = 0;
@@ -414,7 +424,7 @@ static int gen_requirements(const orc_ccsr *g, uint32_t *requirements, uint32_t 
         if (mappingData[currentIndex] > 0) {
             uint32_t relation = g->rel[rowOff + 1 + k];
             /* reflected row: prefer the relation of the entry u -> n (ccsr.cpp:152-160) */
-            uint32_t rdeg = g->tgt[currentVertex];
+            uint32_t rdeg = row_len(g, currentVertex);
             for (uint32_t k2 = 0; k2 < rdeg; k2++) {
                 if (g->tgt[currentVertex + 1 + k2] == localVerticeLocs[relativeDepth]) {
                     relation = g->rel[currentVertex + 1 + k2];
@@ -544,7 +554,8 @@ void orc_row_hash(const uint32_t *row, uint32_t width, uint64_t h[2])
  * search and the equal-target run is scanned for the relation. */
 static inline int req_exists(const orc_ccsr *g, uint32_t rowOff, uint32_t rel, uint32_t vertex)
 {
-    uint32_t len = g->tgt[rowOff];
+    uint32_t len = row_len(g, rowOff);
+    if (len == 0) return 0;
     const uint32_t *t = g->tgt + rowOff + 1;
     const uint32_t *r = g->rel + rowOff + 1;
     uint32_t lo = 0, hi = len;
@@ -644,7 +655,8 @@ static void expand(const match_ctx *mc, uint32_t d, uint32_t *m, thread_acc *acc
     uint32_t roff = p->header[d * 3 + 1];
     uint32_t firstQ = p->header[d * 3 + 2];
     uint32_t parent = m[d - firstQ];
-    uint32_t len = g->tgt[parent];
+    uint32_t len = row_len(g, parent);
+    if (len == 0) return;
     if (!g->dup[parent]) {
         uint32_t cur = d, grel = 0;
         int found = 0;

@@ -178,3 +178,55 @@ def test_lean_last_depth_counts(ctx):
                 assert ctx.match(data, q, tile=tile).count == ref, (s, tile)
             assert ctx.match(data, q, digest=True).count == ref
             assert ctx.match(data, q, mem_budget=2 << 20).count == ref
+
+
+def test_directed_self_loops_vs_oracle(ctx):
+    """Directed data and queries in REF mode: one-way pairs, two-way pairs
+    whose directions carry different labels, and self-loops (every vertex has
+    an out-entry).  Queries the reference compiler accepts give the oracle's
+    count, digest and sorted rows; the ones it rejects fail with status 1."""
+    from test_oracle import directed_entries
+
+    rng = np.random.default_rng(97)
+    n = 60
+    ent = directed_entries(rng, n, p_edge=0.12, p_loop=0.1)
+    s, d, l = (np.array(x, np.uint32) for x in zip(*ent))
+    data = G.Graph.from_edges(n, s, d, l, vlabel=[1] * n)
+    und = {}
+    for a, b, _ in ent:
+        if a != b:
+            und.setdefault(a, set()).add(b)
+            und.setdefault(b, set()).add(a)
+    checked = nonzero = rejected = 0
+    for trial in range(300):
+        q = int(rng.integers(3, 6))
+        verts = [int(rng.integers(0, n))]
+        while len(verts) < q:  # a connected vertex set grown at random
+            nb = sorted(set().union(*(und.get(v, set()) for v in verts)) - set(verts))
+            if not nb:
+                break
+            verts.append(int(rng.choice(nb)))
+        if len(verts) < q:
+            continue
+        idx = {v: i for i, v in enumerate(verts)}
+        qent = [(idx[a], idx[b], lab) for a, b, lab in ent if a in idx and b in idx]
+        if {a for a, _, _ in qent} != set(range(q)):
+            continue  # a query vertex without an out-entry has no row
+        qs, qd, ql = (np.array(x, np.uint32) for x in zip(*qent))
+        query = G.Graph.from_edges(q, qs, qd, ql, vlabel=[1] * q)
+        try:
+            ref = oracle_solve(data, query, G.MODE_REF, rows=True)
+        except O.OracleError as e:
+            assert e.code == 1
+            with pytest.raises(G.GfsmError) as ei:
+                ctx.match(data, query)
+            assert ei.value.code == 1
+            rejected += 1
+            continue
+        m = ctx.match(data, query, rows=True, sorted=True, digest=True)
+        assert_same(m, ref)
+        checked += 1
+        nonzero += ref.count > 0
+        if checked >= 40:
+            break
+    assert checked >= 20 and nonzero >= 10 and rejected > 0, (checked, nonzero, rejected)

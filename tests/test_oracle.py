@@ -352,3 +352,85 @@ def test_output_fs_layout(tmp_path):
     assert raw[:8] == struct.pack("<II", 2, 3)
     assert struct.unpack("<6I", raw[8:]) == (3, 1, 4, 1, 5, 9)
     assert np.array_equal(O.fs_read(p), rows)
+
+
+def brute_force_rule(n, data_entries, q, query_entries, order):
+    """The reference's placement rule taken literally, over all injective maps
+    (ccsr.cpp:142-167, HnGPU.cu:76-109, 243-257): in the placement order, each
+    query entry a->b whose target b is placed no later than a (b == a is a
+    self-loop) requires the data entry f(b)->f(a) with the label of b->a, or
+    of a->b when the query has no b->a entry; the root's image needs an entry
+    of its own.  Entries are unique per ordered pair."""
+    dset = set(data_entries)
+    outdeg = [0] * n
+    for a, _, _ in data_entries:
+        outdeg[a] += 1
+    qlab = {(a, b): l for a, b, l in query_entries}
+    pos = {v: i for i, v in enumerate(order)}
+    reqs = [(a, b, qlab.get((b, a), l)) for (a, b), l in qlab.items() if pos[b] <= pos[a]]
+    rows = []
+    for f in itertools.permutations(range(n), q):
+        if outdeg[f[order[0]]] and all((f[b], f[a], l) in dset for a, b, l in reqs):
+            rows.append(list(f))
+    return sorted(rows)
+
+
+def directed_entries(rng, n, p_edge=0.6, p_loop=0.25, nlabels=2):
+    """Random directed entries, unique per ordered pair, with self-loops; every
+    vertex gets an out-entry (a vertex without one has no row)."""
+    ent = []
+    for a in range(n):
+        if rng.random() < p_loop:
+            ent.append((a, a, int(rng.integers(1, nlabels + 1))))
+        for b in range(a + 1, n):
+            if rng.random() < p_edge:
+                way = rng.random()  # both ways (70%, labels drawn apart), else one way
+                if way < 0.85:
+                    ent.append((a, b, int(rng.integers(1, nlabels + 1))))
+                if way < 0.7 or way >= 0.85:
+                    ent.append((b, a, int(rng.integers(1, nlabels + 1))))
+    has_out = {a for a, _, _ in ent}
+    ent += [(a, (a + 1) % n, 1) for a in range(n) if a not in has_out]
+    return ent
+
+
+def test_directed_self_loops_against_rule():
+    """Asymmetric entries, self-loops and two labels (REF mode), on graphs
+    whose vertices all have out-entries: the oracle equals the literal rule on
+    every query the compiler accepts, and rejects the queries whose later
+    vertices have no entry back to placed ones."""
+    rng = np.random.default_rng(23)
+    checked = nonzero = rejected = 0
+    for trial in range(400):
+        n = int(rng.integers(5, 8))
+        ent = directed_entries(rng, n)
+        q = int(rng.integers(2, 5))
+        verts = [int(v) for v in rng.choice(n, q, replace=False)]
+        idx = {v: i for i, v in enumerate(verts)}
+        qent = [(idx[a], idx[b], l) for a, b, l in ent if a in idx and b in idx and rng.random() < 0.9]
+        if {a for a, _, _ in qent} != set(range(q)):
+            continue  # a query vertex without an out-entry has no row
+        und = {(a, b) for a, b, _ in qent} | {(b, a) for a, b, _ in qent}
+        seen, todo = {0}, [0]
+        while todo:
+            x = todo.pop()
+            for y in range(q):
+                if (x, y) in und and y not in seen:
+                    seen.add(y)
+                    todo.append(y)
+        if len(seen) < q:
+            continue  # disconnected: not a query
+        try:
+            qc = O.CCSR(edges_obj(q, qent), O.MODE_REF)
+            order = [int(c) for c in O.Plan(qc).fields()["col"]]
+        except O.OracleError as e:
+            assert e.code == 1
+            rejected += 1
+            continue
+        r = O.solve(edges_obj(n, ent), edges_obj(q, qent), mode=O.MODE_REF, rows=True)
+        want = brute_force_rule(n, ent, q, qent, order)
+        got = sorted(r.rows.tolist()) if r.count else []
+        assert got == want, (trial, ent, qent, order)
+        checked += 1
+        nonzero += bool(want)
+    assert checked >= 20 and nonzero >= 5, (checked, nonzero, rejected)

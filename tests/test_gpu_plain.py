@@ -230,3 +230,41 @@ def test_directed_self_loops_vs_oracle(ctx):
         if checked >= 40:
             break
     assert checked >= 20 and nonzero >= 10 and rejected > 0, (checked, nonzero, rejected)
+
+
+def test_rowless_vertices_follow_the_rule(ctx):
+    """Directed data where some targets have no out-entries (no row; the
+    reference reads past its segment array there): the product keeps them
+    distinct with empty rows, and its sorted rows equal the placement rule
+    taken literally over all injective maps."""
+    from test_oracle import brute_force_rule, directed_entries, edges_obj
+
+    rng = np.random.default_rng(131)
+    checked = with_rowless = 0
+    for trial in range(1500):
+        n = int(rng.integers(5, 8))
+        ent = directed_entries(rng, n, p_edge=0.5, p_loop=0.25, all_rows=False, p_both=0.2)
+        rowless = set(range(n)) - {a for a, _, _ in ent}
+        if not ent:
+            continue
+        q = int(rng.integers(2, 5))
+        verts = [int(v) for v in rng.choice(n, q, replace=False)]
+        idx = {v: i for i, v in enumerate(verts)}
+        qent = [(idx[a], idx[b], lab) for a, b, lab in ent if a in idx and b in idx and rng.random() < 0.9]
+        if {a for a, _, _ in qent} != set(range(q)):
+            continue  # every query vertex needs an out-entry
+        try:
+            order = [int(c) for c in O.Plan(O.CCSR(edges_obj(q, qent), O.MODE_REF)).fields()["col"]]
+        except O.OracleError as e:
+            assert e.code == 1
+            continue
+        s, d, lab = (np.array(x, np.uint32) for x in zip(*ent))
+        qs, qd, ql = (np.array(x, np.uint32) for x in zip(*qent))
+        data = G.Graph.from_edges(n, s, d, lab, vlabel=[1] * n)
+        query = G.Graph.from_edges(q, qs, qd, ql, vlabel=[1] * q)
+        m = ctx.match(data, query, rows=True, sorted=True)
+        got = m.rows.tolist() if m.count else []
+        assert got == brute_force_rule(n, ent, q, qent, order), (trial, ent, qent, order)
+        checked += 1
+        with_rowless += bool(rowless & {b for _, b, _ in ent})
+    assert checked >= 50 and with_rowless >= 20, (checked, with_rowless)

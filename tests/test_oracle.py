@@ -375,22 +375,24 @@ def brute_force_rule(n, data_entries, q, query_entries, order):
     return sorted(rows)
 
 
-def directed_entries(rng, n, p_edge=0.6, p_loop=0.25, nlabels=2):
-    """Random directed entries, unique per ordered pair, with self-loops; every
-    vertex gets an out-entry (a vertex without one has no row)."""
+def directed_entries(rng, n, p_edge=0.6, p_loop=0.25, nlabels=2, all_rows=True, p_both=0.7):
+    """Random directed entries, unique per ordered pair, with self-loops; with
+    all_rows every vertex gets an out-entry (a vertex without one has no row)."""
     ent = []
     for a in range(n):
         if rng.random() < p_loop:
             ent.append((a, a, int(rng.integers(1, nlabels + 1))))
         for b in range(a + 1, n):
             if rng.random() < p_edge:
-                way = rng.random()  # both ways (70%, labels drawn apart), else one way
-                if way < 0.85:
+                way = rng.random()  # both ways (p_both, labels drawn apart), else one way
+                one = (1.0 - p_both) / 2
+                if way < p_both + one:
                     ent.append((a, b, int(rng.integers(1, nlabels + 1))))
-                if way < 0.7 or way >= 0.85:
+                if way < p_both or way >= p_both + one:
                     ent.append((b, a, int(rng.integers(1, nlabels + 1))))
     has_out = {a for a, _, _ in ent}
-    ent += [(a, (a + 1) % n, 1) for a in range(n) if a not in has_out]
+    if all_rows:
+        ent += [(a, (a + 1) % n, 1) for a in range(n) if a not in has_out]
     return ent
 
 

@@ -802,6 +802,200 @@ void orc_result_free(orc_result *r)
     free(r);
 }
 
+/* ------------------------------------------------------------------------ */
+/* Canonical order and the deterministic per-depth cap                       */
+/* ------------------------------------------------------------------------ */
+
+/* The same enumeration as orc_match, in one fixed order: roots by ascending
+ * original id, and at every depth the generator's entries of the generator
+ * relation by ascending original target id.  The product's bounded-work mode
+ * (a per-depth candidate cap, no reference counterpart) is defined over this
+ * order: depth k tests only its first max_depth_cand candidates, and the
+ * match stops the first time any depth would test one more. */
+
+typedef struct { uint32_t rel, id, off; } canon_ent;
+
+static int canon_cmp(const void *a, const void *b)
+{
+    const canon_ent *x = (const canon_ent *)a, *y = (const canon_ent *)b;
+    if (x->rel != y->rel) return x->rel < y->rel ? -1 : 1;
+    if (x->id != y->id) return x->id < y->id ? -1 : 1;
+    return (x->off > y->off) - (x->off < y->off);
+}
+
+int orc_canon_build(const orc_ccsr *g, orc_canon **out)
+{
+    *out = NULL;
+    orc_canon *cn = (orc_canon *)calloc(1, sizeof(orc_canon));
+    uint64_t W = g->size ? g->size : 1;
+    cn->size = g->size;
+    cn->cid = (uint32_t *)calloc(W, sizeof(uint32_t));
+    cn->coff = (uint32_t *)calloc(W, sizeof(uint32_t));
+    cn->crel = (uint32_t *)calloc(W, sizeof(uint32_t));
+    uint64_t nrows = 0;
+    for (uint64_t i = 0; i < g->size; i += (uint64_t)g->tgt[i] + 1) nrows++;
+    uint64_t *rows = (uint64_t *)malloc((nrows ? nrows : 1) * sizeof(uint64_t));
+    nrows = 0;
+    for (uint64_t i = 0; i < g->size; i += (uint64_t)g->tgt[i] + 1) rows[nrows++] = i;
+    if (!cn->cid || !cn->coff || !cn->crel || !rows) {
+        free(rows);
+        orc_canon_free(cn);
+        return ORC_ERR_NOMEM;
+    }
+#ifdef _OPENMP
+#pragma omp parallel
+#endif
+    {
+        canon_ent *buf = NULL;
+        uint32_t bcap = 0;
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 1024)
+#endif
+        for (uint64_t r = 0; r < nrows; r++) {
+            uint64_t o = rows[r];
+            uint32_t len = g->tgt[o];
+            if (len > bcap) {
+                bcap = len;
+                buf = (canon_ent *)realloc(buf, (size_t)bcap * sizeof(canon_ent));
+            }
+            for (uint32_t e = 0; e < len; e++) {
+                buf[e].rel = g->rel[o + 1 + e];
+                buf[e].off = g->tgt[o + 1 + e];
+                buf[e].id = orc_offset_to_id(g, buf[e].off);
+            }
+            qsort(buf, len, sizeof(canon_ent), canon_cmp);
+            for (uint32_t e = 0; e < len; e++) {
+                cn->crel[o + 1 + e] = buf[e].rel;
+                cn->cid[o + 1 + e] = buf[e].id;
+                cn->coff[o + 1 + e] = buf[e].off;
+            }
+        }
+        free(buf);
+    }
+    free(rows);
+    *out = cn;
+    return ORC_OK;
+}
+
+void orc_canon_free(orc_canon *cn)
+{
+    if (!cn) return;
+    free(cn->cid); free(cn->coff); free(cn->crel);
+    free(cn);
+}
+
+/* First entry of relation rel in the (relation, id)-ordered copy of the row
+ * at offset o (len entries), and the length of that run. */
+static uint32_t canon_run(const orc_canon *cn, uint64_t o, uint32_t len, uint32_t rel, uint32_t *n)
+{
+    const uint32_t *r = cn->crel + o + 1;
+    uint32_t lo = 0, hi = len;
+    while (lo < hi) {
+        uint32_t mid = lo + ((hi - lo) >> 1);
+        if (r[mid] < rel) lo = mid + 1; else hi = mid;
+    }
+    uint32_t e = lo;
+    while (e < len && r[e] == rel) e++;
+    *n = e - lo;
+    return lo;
+}
+
+typedef struct {
+    match_ctx mc;
+    const orc_canon *cn;
+    uint64_t cap;             /* candidates per depth (0 = no cap) */
+    uint64_t cand[64];        /* candidates tested per depth (roots at depth 0) */
+    int stop;
+    thread_acc acc;
+} canon_ctx;
+
+/* Relation of the generator of depth d: the requirement at distance firstQ. */
+static uint32_t gen_rel(const orc_plan *p, uint32_t d)
+{
+    uint32_t num = p->header[d * 3 + 0], roff = p->header[d * 3 + 1], firstQ = p->header[d * 3 + 2];
+    uint32_t cur = d;
+    for (uint32_t j = 0; j < num; j++) {
+        cur -= p->requirements[roff + 2 * j];
+        if (d - cur == firstQ) return p->requirements[roff + 2 * j + 1];
+    }
+    return 0;
+}
+
+static void canon_expand(canon_ctx *c, uint32_t d, uint32_t *m)
+{
+    const orc_ccsr *g = c->mc.g;
+    const orc_plan *p = c->mc.p;
+    if (d == p->q) { emit_leaf(&c->mc, m, &c->acc); return; }
+    uint32_t parent = m[d - p->header[d * 3 + 2]];
+    uint32_t len = row_len(g, parent), n = 0;
+    if (len == 0) return;
+    uint32_t b = canon_run(c->cn, parent, len, gen_rel(p, d), &n);
+    for (uint32_t e = b; e < b + n; e++) {
+        if (c->cap && c->cand[d] == c->cap) { c->stop = 1; return; }
+        c->cand[d]++;
+        uint32_t x = c->cn->coff[parent + 1 + e];
+        if (!check_candidate(&c->mc, d, m, x)) continue;
+        m[d] = x;
+        c->acc.level[d]++;
+        canon_expand(c, d + 1, m);
+        if (c->stop) return;
+    }
+}
+
+int orc_match_canon(const orc_ccsr *g, const orc_canon *cn, const orc_ccsr *qg, const orc_plan *p,
+                    uint32_t root_lo, uint32_t root_hi, uint64_t max_depth_cand,
+                    orc_result **out, int *capped, uint64_t *cand)
+{
+    *out = NULL;
+    *capped = 0;
+    uint32_t q = p->q;
+    if (q == 0 || q > 64) return ORC_ERR_QUERY;
+    if (cn->size != g->size) return ORC_ERR_ARG;
+    canon_ctx *c = (canon_ctx *)calloc(1, sizeof(canon_ctx));
+    c->mc.g = g;
+    c->mc.p = p;
+    c->mc.qg = qg;
+    c->cn = cn;
+    c->cap = max_depth_cand;
+    c->acc.level = (uint64_t *)calloc(q, sizeof(uint64_t));
+    /* Root candidates (HnGPU.cu:475-495 takes every row): vertices with a row
+     * that holds an entry of depth 1's generator relation -- the others have
+     * no child and so no embedding -- in ascending id order. */
+    uint32_t r1 = q > 1 ? gen_rel(p, 1) : 0;
+    uint32_t num = p->header[0], roff = p->header[1];
+    uint32_t hi = root_hi < g->count ? root_hi : g->count;
+    for (uint32_t x = root_lo; x < hi && !c->stop; x++) {
+        uint32_t off = g->verticeLocs[x];
+        uint32_t len = row_len(g, off), n = 0;
+        if (len == 0) continue;
+        if (q > 1) {
+            canon_run(cn, off, len, r1, &n);
+            if (n == 0) continue;
+        }
+        if (c->cap && c->cand[0] == c->cap) { c->stop = 1; break; }
+        c->cand[0]++;
+        int ok = 1;  /* hnInit (HnGPU.cu:95-101): depth-0 requirements are self loops */
+        for (uint32_t j = 0; j < num && ok; j++)
+            if (!req_exists(g, off, p->requirements[roff + 2 * j + 1], off)) ok = 0;
+        if (!ok) continue;
+        uint32_t m[64];
+        m[0] = off;
+        c->acc.level[0]++;
+        canon_expand(c, 1, m);
+    }
+    orc_result *res = (orc_result *)calloc(1, sizeof(orc_result));
+    res->width = q;
+    res->count = c->acc.count;
+    res->digest[0] = c->acc.d0;
+    res->digest[1] = c->acc.d1;
+    res->level = c->acc.level;
+    *capped = c->stop;
+    if (cand) memcpy(cand, c->cand, q * sizeof(uint64_t));
+    free(c);
+    *out = res;
+    return ORC_OK;
+}
+
 int orc_solve(const orc_edges *data, const orc_edges *query, uint32_t mode,
               uint32_t root_lo, uint32_t root_hi, int nthreads, uint64_t max_rows,
               orc_result **out)

@@ -15,6 +15,8 @@
  *   orc_plan_build      HnGPU.cu:749-816, ccsr.cpp:98-204 (matching order + requirement lists)
  *   orc_match           HnGPU.cu:76-109 (hnInit), 114-185 (hnGen), 190-262 (hnCheck), 34-52 (reqExists),
  *                       674-699 (hnWrite: CCSR offset -> original id through staggersInfo)
+ *   orc_match_canon     the same enumeration in one fixed order under a per-depth candidate cap
+ *                       (the product's bounded-work mode; no reference counterpart)
  *   orc_check_sound     authsrc/HnFunction.java:182-194, authsrc/HnGraph.java:28-75
  *   orc_fs_write/read   HnGPU.cu:1081-1091, authsrc/HnFunction.java:21-38, 127-146
  *
@@ -126,6 +128,29 @@ int  orc_match_capped(const orc_ccsr *data, const orc_ccsr *query, const orc_pla
                       uint32_t root_lo, uint32_t root_hi, int nthreads, uint64_t max_rows,
                       uint64_t max_leaves, orc_result **out);
 void orc_result_free(orc_result *r);
+
+/* Canonical enumeration order: each row's entries again, ordered by
+ * (relation, original target id).  Built once per data graph. */
+typedef struct orc_canon {
+    uint64_t size;           /* W of the CCSR it was built from                    */
+    uint32_t *cid;           /* [W] original target ids, per row by (relation, id) */
+    uint32_t *coff;          /* [W] the same entries' target row offsets           */
+    uint32_t *crel;          /* [W] their relations                                */
+} orc_canon;
+int  orc_canon_build(const orc_ccsr *data, orc_canon **out);
+void orc_canon_free(orc_canon *cn);
+
+/* Sequential enumeration in canonical order (roots by ascending id; at each
+ * depth the generator segment by ascending target id) under the deterministic
+ * per-depth cap the product offers for bounded work: depth k tests only its
+ * first max_depth_cand candidates (0 = no cap), and enumeration stops the
+ * first time a depth would test one more (*capped = 1; count and digest are
+ * then partial).  Root candidates are the vertices holding an entry of depth
+ * 1's generator relation.  cand[q] (optional) receives the candidates tested
+ * per depth.  No counterpart in the reference. */
+int  orc_match_canon(const orc_ccsr *data, const orc_canon *cn, const orc_ccsr *query,
+                     const orc_plan *plan, uint32_t root_lo, uint32_t root_hi,
+                     uint64_t max_depth_cand, orc_result **out, int *capped, uint64_t *cand);
 
 /* Convenience: edges -> CCSR -> plan -> match in one call. */
 int  orc_solve(const orc_edges *data, const orc_edges *query, uint32_t mode,

@@ -45,6 +45,11 @@ class _Result(C.Structure):
                 ("level", C.POINTER(C.c_uint64))]
 
 
+class _Canon(C.Structure):
+    _fields_ = [("size", C.c_uint64), ("cid", C.POINTER(C.c_uint32)), ("coff", C.POINTER(C.c_uint32)),
+                ("crel", C.POINTER(C.c_uint32))]
+
+
 class OracleError(RuntimeError):
     def __init__(self, code, what):
         super().__init__(f"{what}: oracle error {code}")
@@ -76,6 +81,10 @@ def lib():
         L.orc_match_capped.argtypes = [P(_CCSR), P(_CCSR), P(_Plan), C.c_uint32, C.c_uint32, C.c_int,
                                        C.c_uint64, C.c_uint64, P(P(_Result))]
         L.orc_result_free.argtypes = [P(_Result)]
+        L.orc_canon_build.argtypes = [P(_CCSR), P(P(_Canon))]
+        L.orc_canon_free.argtypes = [P(_Canon)]
+        L.orc_match_canon.argtypes = [P(_CCSR), P(_Canon), P(_CCSR), P(_Plan), C.c_uint32, C.c_uint32,
+                                      C.c_uint64, P(P(_Result)), P(C.c_int), P(C.c_uint64)]
         L.orc_solve.argtypes = [P(_Edges), P(_Edges), C.c_uint32, C.c_uint32, C.c_uint32, C.c_int,
                                 C.c_uint64, P(P(_Result))]
         L.orc_check_sound.argtypes = [P(_Edges), P(_Edges), P(C.c_uint32), C.c_uint64, C.c_uint32]
@@ -260,6 +269,39 @@ def match_capped(data: CCSR, query: CCSR, plan: Plan, max_leaves, root_lo=0, roo
     if rc:
         raise OracleError(rc, "match_capped")
     return _take_result(p)
+
+
+class Canon:
+    """Each data row's entries ordered by (relation, original target id): the
+    canonical enumeration order of match_canon, built once per data graph."""
+
+    def __init__(self, data: CCSR):
+        p = C.POINTER(_Canon)()
+        rc = lib().orc_canon_build(data._p, C.byref(p))
+        if rc:
+            raise OracleError(rc, "canon_build")
+        self._p = p
+        self._data = data
+
+    def __del__(self):
+        if getattr(self, "_p", None):
+            lib().orc_canon_free(self._p)
+            self._p = None
+
+
+def match_canon(data: CCSR, canon: Canon, query: CCSR, plan: Plan, max_depth_cand=0, root_lo=0,
+                root_hi=0xFFFFFFFF):
+    """Sequential enumeration in canonical order under the deterministic
+    per-depth cap (0 = none).  Returns (Result, capped, cand per depth)."""
+    p = C.POINTER(_Result)()
+    capped = C.c_int()
+    q = plan._p.contents.q
+    cand = (C.c_uint64 * max(q, 1))()
+    rc = lib().orc_match_canon(data._p, canon._p, query._p, plan._p, root_lo, root_hi, max_depth_cand,
+                               C.byref(p), C.byref(capped), cand)
+    if rc:
+        raise OracleError(rc, "match_canon")
+    return _take_result(p), bool(capped.value), [int(cand[i]) for i in range(q)]
 
 
 def check_sound(query: Edges, data: Edges, rows: np.ndarray):

@@ -436,3 +436,101 @@ def test_directed_self_loops_against_rule():
         checked += 1
         nonzero += bool(want)
     assert checked >= 20 and nonzero >= 5, (checked, nonzero, rejected)
+
+
+# ---------------------------------------------------------------- canonical order and per-depth cap
+
+def capped_by_levels(n, data_entries, q, query_entries, order, cap, root_lo=0, root_hi=None):
+    """The per-depth cap restated breadth first, independently of the oracle's
+    depth-first search: build every level of candidates in canonical order
+    (roots by id; children of each survivor, in level order, from its
+    generator's entries of the generator relation by ascending target id),
+    number the candidates of each level, and count the leaves whose own and
+    ancestors' numbers are all below the cap.  The match is capped when some
+    level holds more candidates than the cap.  Symmetric simple data only."""
+    adj = {}
+    for a, b, l in data_entries:
+        adj.setdefault(a, {}).setdefault(l, []).append(b)
+    for a in adj:
+        for l in adj[a]:
+            adj[a][l].sort()
+    qlab = {(a, b): l for a, b, l in query_entries}
+    depth = {v: i for i, v in enumerate(order)}
+    reqs = []  # per depth: [(depth of u, relation)]; distance 0 = self loop
+    for d, nq in enumerate(order):
+        r = [(depth[u], qlab.get((u, nq), l)) for (a, u), l in qlab.items() if a == nq and depth[u] <= d]
+        reqs.append(sorted(r, key=lambda t: (d - t[0], t[1])))
+    gen = [None] + [min((t for t in reqs[d] if t[0] < d), key=lambda t: (d - t[0], t[1])) for d in range(1, q)]
+    hi = n if root_hi is None else min(root_hi, n)
+    level = []  # (path, ok) with the candidate's numbers along the path
+    r1 = gen[1][1] if q > 1 else None
+    for x in range(root_lo, hi):
+        if x not in adj or (r1 is not None and not adj[x].get(r1)):
+            continue
+        ok = all(x in adj[x].get(l, []) for du, l in reqs[0])
+        level.append(([x], ok, [len(level)]))
+    capped = len(level) > cap
+    for d in range(1, q):
+        nxt = []
+        for path, ok, nums in level:
+            if not ok:
+                continue
+            gd, gl = gen[d]
+            for t in adj.get(path[gd], {}).get(gl, []):
+                good = t not in path and all(
+                    t in adj.get(t if du == d else path[du], {}).get(l, []) for du, l in reqs[d])
+                nxt.append((path + [t], good, nums + [len(nxt)]))
+        capped |= len(nxt) > cap
+        level = nxt
+    rows = [p for p, ok, nums in level if ok and max(nums) < cap]
+    out = []
+    for p in rows:
+        row = [0] * q
+        for d, v in enumerate(p):
+            row[order[d]] = v
+        out.append(row)
+    return out, capped
+
+
+def test_canonical_cap_against_breadth_first_restatement():
+    """orc_match_canon: without a cap it finds orc_match's embeddings; under a
+    cap its partial count and digest and its capped flag equal the
+    breadth-first restatement's, for caps from 1 up to past every level."""
+    rng = np.random.default_rng(2024)
+    cases = 0
+    for trial in range(40):
+        n = int(rng.integers(5, 11))
+        edges = [(a, b) for a in range(n) for b in range(a + 1, n) if rng.random() < 0.55]
+        if not edges:
+            continue
+        labels = {e: int(rng.integers(1, 3)) for e in edges}
+        ent = [(a, b, labels[(a, b)]) for a, b in edges] + [(b, a, labels[(a, b)]) for a, b in edges]
+        q = int(rng.integers(2, 6))
+        qe = [(i, int(rng.integers(0, i)), int(rng.integers(1, 3))) for i in range(1, q)]
+        if q > 2 and rng.random() < 0.5:
+            a, b = int(rng.integers(0, q)), int(rng.integers(0, q))
+            if a != b and all({a, b} != {x, y} for x, y, _ in qe):
+                qe.append((a, b, int(rng.integers(1, 3))))
+        qent = [(a, b, l) for a, b, l in qe] + [(b, a, l) for a, b, l in qe]
+        dg = O.CCSR(edges_obj(n, ent), O.MODE_REF)
+        qg = O.CCSR(edges_obj(q, qent), O.MODE_REF)
+        p = O.Plan(qg)
+        cn = O.Canon(dg)
+        order = [int(x) for x in p.fields()["col"]]
+        full = O.match(dg, qg, p)
+        r, capped, cand = O.match_canon(dg, cn, qg, p, 0)
+        assert (r.count, r.digest, capped) == (full.count, full.digest, False)
+        rows_all, _ = capped_by_levels(n, ent, q, qent, order, 1 << 40)
+        assert len(rows_all) == full.count
+        lo = int(rng.integers(0, 3))
+        for cap in sorted({1, 2, 3, 5, 8, 13, max(1, max(cand)), max(cand) + 1, 1000}):
+            r, capped, _ = O.match_canon(dg, cn, qg, p, cap, root_lo=lo)
+            want, wcap = capped_by_levels(n, ent, q, qent, order, cap, root_lo=lo)
+            dg0 = dg1 = 0
+            for row in want:
+                h = O.row_hash(row)
+                dg0, dg1 = (dg0 + h[0]) % (1 << 64), (dg1 + h[1]) % (1 << 64)
+            assert (r.count, capped) == (len(want), wcap), (trial, cap, r.count, len(want), capped, wcap)
+            assert r.digest == (dg0, dg1)
+            cases += 1
+    assert cases > 150

@@ -1,17 +1,24 @@
 """Benchmark: the matcher on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
-                    [--sections plain,batch,serve] [--serve-queries N]
+                    [--sections plain,grid,batch,serve] [--serve-queries N]
 
 plain (the headline `value`): a step is one pass of the single-query matcher
 over the 20 queries of the power-law workload (R-MAT scale 18, edge factor 8,
-16 vertex labels Zipf(1.0), 1 edge label, labelled mode), each restricted to
-its fixed root window (the last K root candidates by id; see
-gfsm_amd/workloads.py), counting embeddings on the device.  The data graph is
-resident in HBM before the timed region.  Also reported: the roofline of the
-dominant kernel (the expand kernel: algorithmic bytes per launch / mean launch
-time, timed with HIP events on the matcher's stream inside the timed region)
-and the CPU oracle on a bounded sample of the same workload on this host.
+16 vertex labels Zipf(1.0), 1 edge label, labelled mode), each over three
+root windows drawn across the degree range (see workloads.config2_sample):
+the lowest-id roots (R-MAT hubs) and the roots from the median on, both under
+the deterministic per-depth candidate cap, and the fixed tail window, exact;
+embeddings counted on the device.  The data graph is resident in HBM before
+the timed region.  Reported beside it: rates per stratum, the share of the
+workload's roots and depth-1 candidates the sample covers, the roofline of
+the dominant kernel (the expand kernel: algorithmic bytes per launch / mean
+launch time, timed with HIP events on the matcher's stream inside the timed
+region), and the CPU oracle on complete windows of the same sample on this
+host's cores.
+
+grid: the road-like workload (1400 x 1400 damaged grid; C4, C6, P7, 3x3
+patch over every root), per-query embeddings/s and the expand roofline.
 
 batch: the 256 queries of the batched workload (R-MAT scale 22) over their
 root windows, once one at a time on the single-query path and once in one
@@ -41,8 +48,7 @@ def parse():
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU-oracle baseline")
-    p.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample budget")
-    p.add_argument("--sections", default="plain,batch,serve",
+    p.add_argument("--sections", default="plain,grid,batch,serve",
                    help="comma-separated sections to run (plain is the headline value)")
     p.add_argument("--serve-queries", type=int, default=10000, help="length of the served stream")
     p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -67,7 +73,8 @@ def calibrate_copy(mib):
 
 
 def traffic_from_profiles():
-    """HBM bytes per expand launch measured with rocprofv3 FETCH_SIZE/WRITE_SIZE
+    """HBM bytes per expand launch of the timed sample, measured with rocprofv3
+    FETCH_SIZE/WRITE_SIZE and scaled by the gather calibration
     (profiles/expand_config2_traffic.json), or None when not recorded."""
     path = os.path.join(ROOT, "profiles", "expand_config2_traffic.json")
     try:
@@ -77,31 +84,82 @@ def traffic_from_profiles():
         return None
 
 
-def cpu_baseline(data, queries, windows, budget_s):
-    """The CPU oracle on a bounded sample of one pass: each query's window in
-    turn, each stopped after at most 2e7 embeddings, until budget_s of
-    matching has been spent (embeddings/s on this host's cores)."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def median_time(fn, runs):
+    ts = []
+    out = None
+    for _ in range(runs):
+        t = time.perf_counter()
+        out = fn()
+        ts.append(time.perf_counter() - t)
+    ts.sort()
+    return ts[len(ts) // 2], out
+
+
+# Complete tail windows of the timed sample that the CPU oracle finishes in a
+# few seconds on all cores: the CPU baseline runs exactly these windows to
+# completion (query indices of the power-law workload).
+CPU_SAMPLE_QUERIES = (0, 6, 11, 12, 18, 19)
+
+
+def cpu_baseline(data, queries, sample, gpu_window_s, runs=5):
+    """The CPU oracle on this host: the same complete windows as the GPU line
+    (CPU_SAMPLE_QUERIES' tail windows), all cores and one thread, median of
+    `runs`; plus the reference's host-side costs, text parse and CCSR build
+    (HnHelper.cuh:496, main.cu:31-46), timed on the oracle's restatement of
+    them and on the product's parser."""
+    import tempfile
+
+    import gfsm_amd as G
+
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
-    V, s, d, l, vl = data.edges()
-    od = O.CCSR(O.Edges.from_arrays(V, s, d, l, vl), O.MODE_VLABEL)
-    threads = min(os.cpu_count() or 1, 16)
-    emb, spent, done = 0, 0.0, 0
-    for q, (lo, hi) in zip(queries, windows):
-        qV, qs, qd, ql, qvl = q.edges()
+    cores = os.cpu_count() or 1
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "data.g")
+        data.write(path)
+        t_parse, edges = median_time(lambda: O.Edges.load(path), runs)
+        t_build, od = median_time(lambda: O.CCSR(edges, O.MODE_VLABEL), runs)
+        t_product, _ = median_time(lambda: G.Graph.load(path, mode=G.MODE_VLABEL), runs)
+    wins = [(i, lo, hi) for i, st, lo, hi, cap in sample if st == "tail" and i in CPU_SAMPLE_QUERIES]
+    plans = []
+    for i, lo, hi in wins:
+        qV, qs, qd, ql, qvl = queries[i].edges()
         qc = O.CCSR(O.Edges.from_arrays(qV, qs, qd, ql, qvl), O.MODE_VLABEL)
-        p = O.Plan(qc)
-        t = time.perf_counter()
-        r = O.match_capped(od, qc, p, 20_000_000, root_lo=lo, root_hi=hi, threads=threads)
-        spent += time.perf_counter() - t
-        emb += r.count
-        done += 1
-        if spent >= budget_s:
-            break
-    return {"value": emb / spent if spent else 0.0, "unit": "embeddings/s", "cores": threads, "kind": "port",
-            "sample": f"first {done} of the 20 windowed queries, each capped at 2e7 embeddings "
-                      f"({emb} embeddings in {spent:.1f} s, OpenMP)"}
+        plans.append((qc, O.Plan(qc), lo, hi))
+
+    def run(threads):
+        return sum(O.match(od, qc, p, root_lo=lo, root_hi=hi, threads=threads).count for qc, p, lo, hi in plans)
+
+    t_all, emb = median_time(lambda: run(cores), runs)
+    t_one, emb1 = median_time(lambda: run(1), runs)
+    assert emb1 == emb
+    gpu_s = sum(gpu_window_s[(i, "tail")] for i, _, _ in wins)
+    return {
+        "value": emb / t_all, "unit": "embeddings/s", "cores": cores, "kind": "port",
+        "single_thread_value": emb / t_one,
+        "cpu_model": cpu_model(),
+        "sample": f"tail windows of queries {list(CPU_SAMPLE_QUERIES)} of the timed sample, each run to completion "
+                  f"({emb} embeddings); median of {runs} runs on all {cores} host cores (OpenMP over roots) and "
+                  f"on one thread",
+        "all_cores_s": t_all, "single_thread_s": t_one,
+        "gpu_same_windows_s": gpu_s,
+        "gpu_vs_all_cores": t_all / gpu_s if gpu_s else None,
+        "host_parse_s": {"oracle_text_parse": t_parse, "oracle_ccsr_build": t_build,
+                         "product_parse_and_layout": t_product,
+                         "what": "R-MAT scale 18 data graph as t/v/e text, median of 5"},
+    }
 
 
 def pct(xs, p):
@@ -243,6 +301,60 @@ def serve_section(n):
     }
 
 
+def expand_roofline(alg, kms, launches):
+    """Algorithmic bytes per expand launch over the mean launch time."""
+    per_launch = alg / max(launches, 1)
+    mean_ms = kms / max(launches, 1)
+    achieved = per_launch / (mean_ms / 1e3) / 1e9 if mean_ms else 0.0
+    return achieved, mean_ms, per_launch
+
+
+def grid_section(steps):
+    """The road-like workload: C4, C6, P7 and the 3x3 patch over every root of
+    the damaged 1400 x 1400 grid, count mode, single-query path; the levels
+    exceed any fixed buffer, so chunked expansion runs throughout."""
+    import gfsm_amd as G
+    import gfsm_amd.workloads as W
+
+    data = W.config3_data()
+    qs = W.config3_queries()
+    ctx = G.Context(0, mem_budget=64 << 30)
+    ctx.upload(data)
+    out = {}
+    tot_alg = tot_kms = tot_l = 0
+    for name, q in qs.items():
+        ctx.match(data, q)  # warm-up (arena, dense tables)
+        emb = cand = alg = launches = chunks = 0
+        kms = 0.0
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            m = ctx.match(data, q, profile=True)
+            st = m.stats
+            emb, cand = m.count, sum(st["cand"])
+            alg += st["alg_bytes"] - 4 * st["roots"]
+            kms += st["expand_ms"]
+            launches += st["expand_launches"]
+            chunks = st["chunks"]
+        wall = (time.perf_counter() - t0) / steps
+        ach, mean_ms, _ = expand_roofline(alg, kms, launches)
+        out[name] = {"embeddings": emb, "candidates": cand, "ms": wall * 1e3, "embeddings_per_s": emb / wall,
+                     "expand_launches": launches // steps, "chunked_launches": chunks,
+                     "expand_achieved_gbs": ach, "expand_frac": ach / HBM_PEAK_GBS}
+        tot_alg += alg
+        tot_kms += kms
+        tot_l += launches
+    ctx.close()
+    ach, mean_ms, _ = expand_roofline(tot_alg, tot_kms, tot_l)
+    return {
+        "workload": "road-like: 1400 x 1400 grid, 5% of edges deleted, 2% diagonals added, unlabeled; "
+                    "C4, C6, P7, 3x3 patch, every root, count mode (chunked expansion)",
+        "queries": out,
+        "embeddings_per_s": sum(v["embeddings"] for v in out.values()) / sum(v["ms"] / 1e3 for v in out.values()),
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach / HBM_PEAK_GBS, "launches": tot_l // steps, "mean_launch_ms": mean_ms},
+    }
+
+
 def main():
     args = parse()
     if args.calibrate_copy:
@@ -258,43 +370,73 @@ def main():
     sections = set(args.sections.split(","))
     data = W.config2_data()
     queries = W.config2_queries(data)
-    windows = W.config2_windows(data, queries)
+    sample = W.config2_sample(data, queries)
     ctx = G.Context(0, mem_budget=64 << 30)
     ctx.upload(data)
+    strata = W.CONFIG2_STRATA
 
-    def step(profile):
-        emb, cand, kms, launches, alg = 0, 0, 0.0, 0, 0
-        for q, (lo, hi) in zip(queries, windows):
-            m = ctx.match(data, q, root_lo=lo, root_hi=hi, profile=profile)
+    def step(profile, acc):
+        for i, stratum, lo, hi, cap in sample:
+            t = time.perf_counter()
+            m = ctx.match(data, queries[i], root_lo=lo, root_hi=hi, max_depth_cand=cap, allow_partial=True,
+                          profile=profile)
+            dt = time.perf_counter() - t
             st = m.stats
-            emb += m.count
-            cand += sum(st["cand"])
-            kms += st["expand_ms"]
-            launches += st["expand_launches"]
-            alg += st["alg_bytes"] - 4 * st["roots"]  # the root kernel's share is not the expand kernel's
-        return emb, cand, kms, launches, alg
+            a = acc[stratum]
+            a["emb"] += m.count
+            a["cand"] += sum(st["cand"])
+            a["roots"] += st["cand"][0]
+            a["first"] += st["cand"][1] if len(st["cand"]) > 1 else 0
+            a["s"] += dt
+            a["kms"] += st["expand_ms"]
+            a["launches"] += st["expand_launches"]
+            a["alg"] += st["alg_bytes"] - 4 * st["roots"]  # the root kernel's share is not the expand kernel's
+            a["capped"] += m.status == 5
+            win_s[(i, stratum)] = win_s.get((i, stratum), 0.0) + dt
 
+    def fresh():
+        return {s: dict(emb=0, cand=0, roots=0, first=0, s=0.0, kms=0.0, launches=0, alg=0, capped=0) for s in strata}
+
+    win_s = {}
     for _ in range(args.warmup):
-        step(True)
+        step(True, fresh())
+    win_s = {}
+    acc = fresh()
     t0 = time.perf_counter()
-    tot = [0, 0, 0.0, 0, 0]
     for _ in range(args.steps):
-        r = step(True)
-        tot = [a + b for a, b in zip(tot, r)]
+        step(True, acc)
     wall = time.perf_counter() - t0
-    emb, cand, kms, launches, alg = tot
-    per_launch_bytes = alg / max(launches, 1)
-    mean_ms = kms / max(launches, 1)
-    achieved = per_launch_bytes / (mean_ms / 1e3) / 1e9 if mean_ms else 0.0
-    traffic = traffic_from_profiles()
+    K = args.steps
+    win_s = {k: v / K for k, v in win_s.items()}
+    emb = sum(a["emb"] for a in acc.values())
+    alg = sum(a["alg"] for a in acc.values())
+    kms = sum(a["kms"] for a in acc.values())
+    launches = sum(a["launches"] for a in acc.values())
+    achieved, mean_ms, _ = expand_roofline(alg, kms, launches)
+    # coverage of the whole workload (every root of every query)
+    roots_total = first_total = 0
+    for q in queries:
+        nodes, cand = data.projection(q)
+        roots_total += int(cand[0])
+        first_total += int(cand[1])
+    per = {}
+    for sname, a in acc.items():
+        ach, mms, _ = expand_roofline(a["alg"], a["kms"], a["launches"])
+        per[sname] = {"embeddings_per_step": a["emb"] // K, "candidates_per_step": a["cand"] // K,
+                      "ms_per_step": a["s"] * 1e3 / K, "embeddings_per_s": a["emb"] / a["s"] if a["s"] else 0.0,
+                      "candidates_per_s": a["cand"] / a["s"] if a["s"] else 0.0,
+                      "expand_achieved_gbs": ach, "expand_frac": ach / HBM_PEAK_GBS,
+                      "roots_tested": a["roots"] // K, "capped_windows": a["capped"] // K}
+    roots_tested = sum(a["roots"] for a in acc.values()) // K
+    first_tested = sum(a["first"] for a in acc.values()) // K
     line = {
         "metric": "embeddings/s + queries/s on 1 MI355X; p50/p99 latency, served vs serial",
         "value": emb / wall,
         "unit": "embeddings/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": K,
         "warmup": args.warmup,
-        "ms_per_step": wall * 1e3 / args.steps,
+        "ms_per_step": wall * 1e3 / K,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -302,30 +444,40 @@ def main():
         "data": "synthetic (seeded R-MAT graph and random-walk queries)",
         "config": {
             "workload": "power-law plain single query: R-MAT scale 18, edge factor 8, 16 vertex labels "
-                        "Zipf(1.0), 1 edge label; 20 random-walk queries of 8 vertices (10 sparse, 10 dense), "
-                        "each over its fixed root window; count mode",
+                        "Zipf(1.0), 1 edge label; 20 random-walk queries of 8 vertices (10 sparse, 10 dense), each "
+                        f"over a stratified root sample: the {W.CONFIG2_STRATUM_ROOTS} lowest-id roots (hub end) and "
+                        f"{W.CONFIG2_STRATUM_ROOTS} roots from the median root on, both under the deterministic "
+                        f"per-depth cap of {W.CONFIG2_STRATUM_CAP} candidates, and the fixed tail window (the last "
+                        "1-256 roots by id), exact; count mode",
             "queries_per_step": len(queries),
-            "embeddings_per_step": emb // args.steps,
-            "candidates_per_step": cand // args.steps,
+            "windows_per_step": len(sample),
+            "embeddings_per_step": emb // K,
+            "candidates_per_step": sum(a["cand"] for a in acc.values()) // K,
+            "coverage": {"roots_tested": roots_tested, "roots_total": roots_total,
+                         "roots_frac": roots_tested / roots_total,
+                         "depth1_candidates_tested": first_tested, "depth1_candidates_total": first_total,
+                         "depth1_frac": first_tested / first_total,
+                         "note": "of all 20 queries over all their roots; hub and middle windows test only a "
+                                 "capped prefix of their roots' subtrees"},
+            "strata": per,
         },
-        "queries_per_s": len(queries) * args.steps / wall,
+        "queries_per_s": len(sample) * K / wall,
         "roofline": {
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
+            "traffic": traffic_from_profiles(),
             "kernel": "gfsm::k_expand (inner depths) + gfsm::k_expand_bare (last depth)",
-            "launches": launches // args.steps,
+            "launches": launches // K,
             "mean_launch_ms": mean_ms,
         },
     }
     ctx.close()
-    if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(data, queries, windows, args.cpu_seconds)
-    else:
-        line["cpu_baseline"] = None
+    line["cpu_baseline"] = None if args.no_cpu else cpu_baseline(data, queries, sample, win_s)
+    if "grid" in sections:
+        line["grid"] = grid_section(max(1, min(K, 3)))
     if "batch" in sections:
         line["batched"] = batch_section()
     if "serve" in sections:

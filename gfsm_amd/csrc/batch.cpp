@@ -96,6 +96,11 @@ uint64_t RegionAlloc::largest() const
 
 static uint64_t node_bytes(uint32_t k) { return 4ull * (k + 1) + 12; }
 
+void BatchEngine::set_max_region(uint64_t b)
+{
+    max_region_ = std::max<uint64_t>(kRegionAlign, b / kRegionAlign * kRegionAlign);
+}
+
 // Smallest region whose equal per-level share still gives every level
 // kMinLevelNodes nodes after carve()'s slack.
 uint64_t BatchEngine::min_region(uint32_t q) const

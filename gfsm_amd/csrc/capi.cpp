@@ -168,8 +168,7 @@ gfsm_status gfsm_graph_upload(gfsm_ctx *ctx, const gfsm_graph *g)
 void gfsm_graph_free(gfsm_graph *g)
 {
     if (!g) return;
-    release_graph_mirrors(*g->g);
-    delete g->g;
+    free_graph_when_unpinned(g->g);  // a running server may still use it
     delete g;
 }
 

@@ -128,6 +128,14 @@ static void free_mirror(DevGraph *dg)
 void ctx_close(Ctx *ctx)
 {
     if (!ctx) return;
+    {
+        std::vector<void *> running;
+        {
+            std::lock_guard<std::mutex> lk(ctx->srv_mu);
+            running.assign(ctx->servers.begin(), ctx->servers.end());
+        }
+        for (void *s : running) ctx->halt_server(s);
+    }
     (void)hipSetDevice(ctx->device);
     ctx->drop_batch_cache();
     {
@@ -157,6 +165,41 @@ void release_graph_mirrors(const HostGraph &g)
         free_mirror(kv.second);
     }
     g.dev.clear();
+}
+
+static std::mutex g_pin_mu;  // guards HostGraph::pins and free_pending
+
+void pin_graph(const HostGraph &g)
+{
+    std::lock_guard<std::mutex> lk(g_pin_mu);
+    g.pins++;
+}
+
+void unpin_graph(const HostGraph &g)
+{
+    bool free_now = false;
+    {
+        std::lock_guard<std::mutex> lk(g_pin_mu);
+        if (g.pins) g.pins--;
+        free_now = g.pins == 0 && g.free_pending;
+    }
+    if (free_now) {
+        release_graph_mirrors(g);
+        delete &g;
+    }
+}
+
+void free_graph_when_unpinned(HostGraph *g)
+{
+    {
+        std::lock_guard<std::mutex> lk(g_pin_mu);
+        if (g->pins) {
+            g->free_pending = true;
+            return;
+        }
+    }
+    release_graph_mirrors(*g);
+    delete g;
 }
 
 Status ensure_arena(Arena &a, uint64_t bytes)
@@ -242,6 +285,13 @@ Status plan_for(const HostGraph &data, const HostGraph &query, const gfsm_match_
     if (o.root_lo || o.root_hi)
         return Status::err(GFSM_ERR_ARG, "root windows refer to the reference order's root; not with GFSM_ORDER_COST");
     return plan_query(query, plan, &data);
+}
+
+const uint2 *dense_segments_cached(DevGraph &dg, uint32_t rel)
+{
+    std::lock_guard<std::mutex> lk(dg.dense_mu);
+    auto it = dg.dense.find(rel);
+    return it != dg.dense.end() ? it->second : nullptr;
 }
 
 const uint2 *dense_segments(const HostGraph &data, DevGraph &dg, uint32_t rel, uint64_t lookups, hipStream_t s)
@@ -378,8 +428,23 @@ struct Runner {
     uint64_t streamed = 0, sdg[2] = {0, 0};
     std::vector<uint32_t> stage;
 
-    Runner(Ctx &c, const HostGraph &d, const DevGraph &g, const Plan &p, uint32_t f, uint64_t mr, Result &r)
-        : ctx(c), data(d), dg(g), plan(p), flags(f), max_rows(mr), res(r) {}
+    // The lane (stream, scan workspace, totals); optionally a fixed level
+    // arena, with which nothing is allocated (dense tables only if cached).
+    Lane &lane;
+    const Arena *fixed = nullptr;
+    bool no_alloc = false;
+
+    Runner(Ctx &c, const HostGraph &d, const DevGraph &g, const Plan &p, uint32_t f, uint64_t mr, Result &r,
+           Lane &l)
+        : ctx(c), data(d), dg(g), plan(p), flags(f), max_rows(mr), res(r), lane(l) {}
+
+    // Dense segment table of `rel` for a step of `lookups` lookups: built on
+    // demand, or, without allocation, only one an earlier match built.
+    const uint2 *dense(uint32_t rel, uint64_t lookups)
+    {
+        DevGraph &g = const_cast<DevGraph &>(dg);
+        return no_alloc ? dense_segments_cached(g, rel) : dense_segments(data, g, rel, lookups, lane.stream);
+    }
 
     // Does the expand of depth d stage the next generator's segment per node?
     // (Only when the next generator is an older ancestor, not the new vertex.)
@@ -399,7 +464,7 @@ struct Runner {
     // must fit the 46-bit value field of the scan words.
     uint64_t launch_limit() const
     {
-        const uint64_t by_ws = ctx.max_tiles * kTile;  // enough tiles at any tile size
+        const uint64_t by_ws = lane.ws.max_tiles * kTile;  // enough tiles at any tile size
         const uint64_t by_val = kValueMask / std::max<uint64_t>(1, data.max_degree);
         return std::max<uint64_t>(kTile * 8, std::min(by_ws, by_val));
     }
@@ -438,9 +503,18 @@ struct Runner {
             chunk[k + 1] = std::min<uint64_t>(c, launch_limit());
         }
         chunk[q - 1] = launch_limit();
-        Status st = ensure_arena(ctx, total);
-        if (!st.good()) return st;
-        char *p = ctx.arena.base;
+        char *p = nullptr;
+        if (fixed) {
+            if (total > fixed->bytes)
+                return Status::err(GFSM_ERR_NOMEM, "query needs " + std::to_string(total) +
+                                                       " level bytes; the fixed arena holds " +
+                                                       std::to_string(fixed->bytes));
+            p = fixed->base;
+        } else {
+            Status st = ensure_arena(ctx, total);
+            if (!st.good()) return st;
+            p = ctx.arena.base;
+        }
         auto take = [&p](uint64_t b) {
             char *r = p;
             p += round256(b);
@@ -471,9 +545,9 @@ struct Runner {
 
     Status sync_totals()
     {
-        HIP_TRY(hipMemcpyAsync(ctx.lane.tot_h, ctx.lane.tot_d, sizeof(Totals), hipMemcpyDeviceToHost,
-                               ctx.lane.stream));
-        HIP_TRY(hipStreamSynchronize(ctx.lane.stream));
+        HIP_TRY(hipMemcpyAsync(lane.tot_h, lane.tot_d, sizeof(Totals), hipMemcpyDeviceToHost,
+                               lane.stream));
+        HIP_TRY(hipStreamSynchronize(lane.stream));
         return Status::ok();
     }
 
@@ -486,26 +560,26 @@ struct Runner {
     Status kbegin()
     {
         if (!prof) return Status::ok();
-        std::vector<hipEvent_t> &ev = ctx.lane.kev;
+        std::vector<hipEvent_t> &ev = lane.kev;
         while (ev.size() < 2ull * (kn + 1)) {
             hipEvent_t e = nullptr;
             HIP_TRY(hipEventCreate(&e));
             ev.push_back(e);
         }
-        HIP_TRY(hipEventRecord(ev[2 * kn], ctx.lane.stream));
+        HIP_TRY(hipEventRecord(ev[2 * kn], lane.stream));
         return Status::ok();
     }
     Status kend(bool expand)
     {
         if (!prof) return Status::ok();
-        HIP_TRY(hipEventRecord(ctx.lane.kev[2 * kn + 1], ctx.lane.stream));
+        HIP_TRY(hipEventRecord(lane.kev[2 * kn + 1], lane.stream));
         kexp[kn++] = expand;
         return kn == kProfPairs ? prof_flush() : Status::ok();
     }
     Status prof_flush()
     {
         if (kn == 0) return Status::ok();
-        const std::vector<hipEvent_t> &ev = ctx.lane.kev;
+        const std::vector<hipEvent_t> &ev = lane.kev;
         HIP_TRY(hipEventSynchronize(ev[2 * kn - 1]));
         for (uint32_t i = 0; i < kn; i++) {
             float ms = 0;
@@ -541,25 +615,25 @@ struct Runner {
             a.io.out_off = off[0];
             a.io.out_gbeg = gbeg[0];
             a.io.cap = cap[0];
-            a.io.tot = ctx.lane.tot_d;
+            a.io.tot = lane.tot_d;
             a.io.ntiles = (a.io.n + kRootTile - 1) / kRootTile;
             a.nchk = plan.dev.nroot_chk;
             std::memcpy(a.chk_rel, plan.dev.root_chk_rel, sizeof(a.chk_rel));
-            a.ws = ctx.lane.ws;
-            if (a.io.ntiles > ctx.max_tiles || e > data.rel_vtx.size())
+            a.ws = lane.ws;
+            if (a.io.ntiles > lane.ws.max_tiles || e > data.rel_vtx.size())
                 return Status::err(GFSM_ERR_ARG, "internal: root launch shape check failed");
-            HIP_TRY(ctx.lane.before_launch());
+            HIP_TRY(lane.before_launch());
             {
                 Status ks = kbegin();
                 if (!ks.good()) return ks;
             }
-            HIP_TRY(launch_root(a, grid_for(a.io.ntiles), ctx.lane.stream));
+            HIP_TRY(launch_root(a, grid_for(a.io.ntiles), lane.stream));
             res.stats.launches++;
             Status st = kend(false);
             if (!st.good()) return st;
             st = sync_totals();
             if (!st.good()) return st;
-            const uint64_t S0 = ctx.lane.tot_h->keep, G1 = ctx.lane.tot_h->cnt;
+            const uint64_t S0 = lane.tot_h->keep, G1 = lane.tot_h->cnt;
             res.stats.level[0] += S0;
             res.stats.cand[0] += e - c;  // root candidates tested
             depth_done[0] += e - c;
@@ -609,11 +683,9 @@ struct Runner {
             StepIO &io = a.io;
             io.need_nxt = need_nxt(kind, d);
             if (kind == EXPAND_MID)
-                a.dp.nxt_seg = dense_segments(data, const_cast<DevGraph &>(dg), plan.dev.depth[d].nxt_rel, e - c,
-                                              ctx.lane.stream);
+                a.dp.nxt_seg = dense(plan.dev.depth[d].nxt_rel, e - c);
             for (uint32_t r = 0; r < std::min<uint32_t>(plan.dev.depth[d].nchk, kDenseChk); r++)
-                a.dp.chk_seg[r] = dense_segments(data, const_cast<DevGraph &>(dg), plan.dev.depth[d].chk_rel[r],
-                                                 e - c, ctx.lane.stream);
+                a.dp.chk_seg[r] = dense(plan.dev.depth[d].chk_rel[r], e - c);
             io.d = d;
             io.q = q;
             io.kind = kind;
@@ -633,12 +705,12 @@ struct Runner {
                 io.rows = rows_d;
                 io.cap = rows_cap;
             }
-            io.tot = ctx.lane.tot_d;
+            io.tot = lane.tot_d;
             io.ntiles = (e - c + tc - 1) / tc;
-            a.ws = ctx.lane.ws;
+            a.ws = lane.ws;
             // Launch-shape checks: the workspace holds every tile, the frontier
             // is non-empty and the LDS staging fits.
-            if (io.ntiles == 0 || io.ntiles > ctx.max_tiles || S_prev == 0 || S_prev > cap[d - 1] ||
+            if (io.ntiles == 0 || io.ntiles > lane.ws.max_tiles || S_prev == 0 || S_prev > cap[d - 1] ||
                 a.dp.gen_dist == 0 || a.dp.gen_dist > d || expand_lds_bytes(items, pool_words) > kMaxDynLds)
                 return Status::err(GFSM_ERR_ARG, "internal: expand launch shape check failed");
             if (max_cand && cand_total + (e - c) > max_cand)
@@ -647,17 +719,17 @@ struct Runner {
             // The tiles' frontier nodes, searched by the whole GPU ahead of the
             // step; a step of a few tiles searches inside its tiles instead of
             // paying for another launch.
-            if (ctx.lane.tile_node && io.ntiles >= kTileIndexMin) {
-                HIP_TRY(launch_tile_index(io.in_off, S_prev, c, e, tc, io.ntiles, ctx.lane.tile_node, ctx.lane.stream));
-                io.tile_node = ctx.lane.tile_node;
+            if (lane.tile_node && io.ntiles >= kTileIndexMin) {
+                HIP_TRY(launch_tile_index(io.in_off, S_prev, c, e, tc, io.ntiles, lane.tile_node, lane.stream));
+                io.tile_node = lane.tile_node;
             }
-            HIP_TRY(ctx.lane.before_launch());
+            HIP_TRY(lane.before_launch());
             {
                 Status ks = kbegin();
                 if (!ks.good()) return ks;
             }
             HIP_TRY(launch_expand(a, (uint32_t)std::min<uint64_t>(io.ntiles, (uint64_t)grid_cap * grid_mul),
-                                  ctx.lane.stream));
+                                  lane.stream));
             res.stats.launches++;
             if (!first) res.stats.chunks++;
             first = false;
@@ -671,7 +743,7 @@ struct Runner {
             }
             st = sync_totals();
             if (!st.good()) return st;
-            const uint64_t keep = ctx.lane.tot_h->keep;
+            const uint64_t keep = lane.tot_h->keep;
             if (keep > e - c) return Status::err(GFSM_ERR_DEVICE, "internal: more nodes kept than candidates");
             const uint64_t limit = (kind == EXPAND_MID) ? cap[d] : rows_cap;
             if (keep > limit) {
@@ -687,13 +759,13 @@ struct Runner {
                 const double est = (double)limit * (double)(e - c) / (double)keep * 0.7;
                 chunk[d] = (uint64_t)std::max<double>((double)tc, std::min<double>(est, (double)tile_limit));
                 if (max_depth_cand && kind == EXPAND_MID)
-                    chunk[d] = capped_chunk(d, e - c, ctx.lane.tot_h->cnt, chunk[d], tc);
+                    chunk[d] = capped_chunk(d, e - c, lane.tot_h->cnt, chunk[d], tc);
             } else {
                 chunk[d] = std::min<uint64_t>(chunk[d] * 4, tile_limit);
             }
             if (kind == EXPAND_MID) {
                 if (keep) {
-                    st = descend(d + 1, 0, ctx.lane.tot_h->cnt, keep);
+                    st = descend(d + 1, 0, lane.tot_h->cnt, keep);
                     if (!st.good()) return st;
                 }
             } else {
@@ -768,30 +840,18 @@ Status selftest_scan(Ctx &ctx, const uint32_t *vals, uint64_t n, uint64_t *prefi
     return st;
 }
 
-Status match_plain(Ctx &ctx, const HostGraph &data, const HostGraph &query, const gfsm_match_opts &o, Result &res,
-                   FILE *stream)
-{
-    const double t0 = now_ms();
-    res = Result{};
-    if (data.mode != query.mode) return Status::err(GFSM_ERR_ARG, "query and data graph use different modes");
-    Plan plan;
-    Status st = plan_for(data, query, o, plan);
-    if (!st.good()) return st;
-    res.width = plan.q;
-    res.stats.q = plan.q;
-    if (plan.q == 1) {
-        st = match_single_vertex(data, plan, o, res);
-        res.stats.host_ms = now_ms() - t0;
-        return st;
-    }
-    std::lock_guard<std::mutex> lk(ctx.mu);
-    HIP_TRY(hipSetDevice(ctx.device));
-    DevGraph *dg = nullptr;
-    st = upload_graph(ctx, data, dg);
-    if (!st.good()) return st;
+namespace {
 
-    Runner run(ctx, data, *dg, plan, o.flags, o.max_rows, res);
+// One single-query match on `L` with the data graph resident: the depth-first
+// chunk walk from the roots, then the result.  With `fixed`, the levels live
+// in that arena and nothing is allocated (count mode only).
+Status run_match(Ctx &ctx, const HostGraph &data, const DevGraph &dg, const Plan &plan, const gfsm_match_opts &o,
+                 Result &res, FILE *stream, Lane &L, const Arena *fixed)
+{
+    Runner run(ctx, data, dg, plan, o.flags, o.max_rows, res, L);
     run.stream = stream;
+    run.fixed = fixed;
+    run.no_alloc = fixed != nullptr;
     run.max_cand = o.max_cand;
     run.max_depth_cand = o.max_depth_cand;
     if (o.tile) {
@@ -799,7 +859,8 @@ Status match_plain(Ctx &ctx, const HostGraph &data, const HostGraph &query, cons
             return Status::err(GFSM_ERR_ARG, "tile must be 0, 256, 512, 1024 or 2048 candidates");
         run.forced_items = o.tile / kTile;
     }
-    st = run.carve(o.mem_budget ? o.mem_budget : ctx.budget);
+    const uint64_t budget = o.mem_budget ? o.mem_budget : ctx.budget;
+    Status st = run.carve(fixed ? std::min(budget, fixed->bytes) : budget);
     if (!st.good()) return st;
     if (o.max_depth_cand)  // capped: start with an eighth of the cap, then follow the observed fan-out
         for (uint32_t k = 0; k < GFSM_MAXQ; k++)
@@ -807,8 +868,15 @@ Status match_plain(Ctx &ctx, const HostGraph &data, const HostGraph &query, cons
     uint64_t rlo, rhi;
     root_range(data, plan, o.root_lo, o.root_hi, rlo, rhi);
 
-    Lane &L = ctx.lane;
-    HIP_TRY(hipMemsetAsync(L.tot_d, 0, sizeof(Totals), L.stream));
+    if (fixed) {
+        // The workspace in the shared array carries the previous call's words:
+        // clear the tile words, the control block and the totals (fill
+        // kernels, so the sequence holds no memset node).
+        HIP_TRY(launch_ws_clear(L.ws, L.stream));
+        HIP_TRY(launch_ws_reset(L.ws, L.tot_d, L.stream));
+    } else {
+        HIP_TRY(hipMemsetAsync(L.tot_d, 0, sizeof(Totals), L.stream));
+    }
     HIP_TRY(hipEventRecord(L.ev[0], L.stream));
     st = run.run_roots(rlo, rhi);
     Status capped = Status::ok();
@@ -855,8 +923,35 @@ Status match_plain(Ctx &ctx, const HostGraph &data, const HostGraph &query, cons
     }
     for (uint32_t k = 1; k < q; k++) res.stats.checks[k] = plan.dev.depth[k].nchk;
     res.stats.alg_bytes = alg_bytes(plan, res.stats, o.flags);
-    res.stats.host_ms = now_ms() - t0;
     return capped;
+}
+
+}  // namespace
+
+Status match_plain(Ctx &ctx, const HostGraph &data, const HostGraph &query, const gfsm_match_opts &o, Result &res,
+                   FILE *stream)
+{
+    const double t0 = now_ms();
+    res = Result{};
+    if (data.mode != query.mode) return Status::err(GFSM_ERR_ARG, "query and data graph use different modes");
+    Plan plan;
+    Status st = plan_for(data, query, o, plan);
+    if (!st.good()) return st;
+    res.width = plan.q;
+    res.stats.q = plan.q;
+    if (plan.q == 1) {
+        st = match_single_vertex(data, plan, o, res);
+        res.stats.host_ms = now_ms() - t0;
+        return st;
+    }
+    std::lock_guard<std::mutex> lk(ctx.mu);
+    HIP_TRY(hipSetDevice(ctx.device));
+    DevGraph *dg = nullptr;
+    st = upload_graph(ctx, data, dg);
+    if (!st.good()) return st;
+    st = run_match(ctx, data, *dg, plan, o, res, stream, ctx.lane, nullptr);
+    res.stats.host_ms = now_ms() - t0;
+    return st;
 }
 
 }  // namespace gfsm

@@ -35,6 +35,8 @@ struct DevGraph {
 // lookups of it, within kDenseMaxBytes per mirror.  Null: use the hash table.
 const uint2 *dense_segments(const HostGraph &data, DevGraph &dg, uint32_t rel, uint64_t lookups, hipStream_t s);
 constexpr uint64_t kDenseMaxBytes = 4ull << 30;
+// The table of `rel` if one was built already, else null (never builds).
+const uint2 *dense_segments_cached(DevGraph &dg, uint32_t rel);
 
 // A stream with its own chained-scan workspace and result block.  Kernels of
 // one lane run in stream order, so the workspace is never shared by two
@@ -94,6 +96,12 @@ struct Ctx {
     void *batch_cache = nullptr;
     const HostGraph *batch_graph = nullptr;
     void (*batch_cache_free)(void *) = nullptr;
+    // Servers running on this context: closing the context stops each of
+    // them first (it finishes the work already submitted), so no server
+    // outlives the device memory it uses.  Owned by serve.cpp.
+    std::mutex srv_mu;
+    std::set<void *> servers;
+    void (*halt_server)(void *) = nullptr;
     void drop_batch_cache()
     {
         if (batch_cache && batch_cache_free) batch_cache_free(batch_cache);
@@ -108,6 +116,11 @@ Status ensure_arena(Ctx &ctx, uint64_t bytes);
 Status ensure_arena(Arena &a, uint64_t bytes);
 Status upload_graph(Ctx &ctx, const HostGraph &g, DevGraph *&out);
 void release_graph_mirrors(const HostGraph &g);
+// Server pins on a graph (see HostGraph::pins).  free_graph_when_unpinned
+// frees the graph now, or marks it and leaves it to the last unpin.
+void pin_graph(const HostGraph &g);
+void unpin_graph(const HostGraph &g);
+void free_graph_when_unpinned(HostGraph *g);
 
 // Plain single-query path: host-driven depth loop with chunked expansion.
 // With `stream`, rows mode writes each chunk's rows to it (after an output.fs

@@ -146,7 +146,7 @@ bool layout_ok(const HostGraph &g)
         for (uint64_t i = b; i < e; i++)
             if (g.nbr[i] >= g.V || g.src[i] >= g.V || g.dst[i] >= g.V) bad = true;
     });
-    if (bad || maxdeg != g.max_degree) return false;
+    if (bad || maxdeg != g.max_degree || g.max_degree >= (1u << 26)) return false;  // build_graph's bound
     for (size_t k = 1; k < g.rel_keys.size(); k++)
         if (g.rel_keys[k - 1] >= g.rel_keys[k]) return false;
     if (g.rel_off.size() != g.rel_keys.size() + 1 || g.rel_off[0] != 0 || g.rel_off.back() != g.rel_vtx.size())
@@ -198,6 +198,11 @@ Status load_bin(const char *path, HostGraph *&out)
     if (h.version != kVersion) return Status::err(GFSM_ERR_INVALID_DATA, "unsupported graph image version");
     if (h.mode != GFSM_MODE_REF && h.mode != GFSM_MODE_VLABEL)
         return Status::err(GFSM_ERR_INVALID_DATA, "bad mode in graph image");
+    // Every count fits 32 bits (the layout's indices are u32) and the degree
+    // bound of build_graph holds, so the size sum below cannot wrap.
+    if (h.ne >= (1ull << 32) || h.nent >= 0xFFFFFFFFull || h.ndir >= (1ull << 32) || h.nrel >= (1ull << 32) ||
+        h.nrv >= (1ull << 32) || h.max_degree >= (1u << 26))
+        return Status::err(GFSM_ERR_INVALID_DATA, "graph image header out of range");
     // Sizes must add up to the file length before anything large is allocated.
     const uint64_t want = sizeof(h) + 4ull * h.V + 12ull * h.ne + 4ull * h.nent + 4ull * ((uint64_t)h.V + 1) +
                           8ull * (h.ndir + 1) + 4ull * h.nrel + 8ull * (h.nrel + 1) + 12ull * h.nrv;

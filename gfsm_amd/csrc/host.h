@@ -2,6 +2,7 @@
 // layout, compiled query plans, results, and the error channel of the C ABI.
 #pragma once
 
+#include <atomic>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -68,6 +69,11 @@ struct HostGraph {
 
     mutable std::mutex dev_mu;
     mutable std::unordered_map<const void *, DevGraph *> dev;  // ctx -> mirror
+    // Running servers that use this graph (as their data graph or as a query
+    // in flight): while pinned, gfsm_graph_free only marks it, and the last
+    // unpin frees it (pin_graph / unpin_graph / free_graph_when_unpinned).
+    mutable uint32_t pins = 0;
+    mutable bool free_pending = false;
     mutable std::mutex stats_mu;
     mutable std::vector<uint64_t> rel_entries;  // per rel_keys index, filled on first use
     mutable std::vector<uint64_t> label_count;  // vertices per vertex label, filled on first use

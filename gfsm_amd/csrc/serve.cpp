@@ -24,12 +24,19 @@ struct Server {
     std::unique_ptr<BatchEngine> eng;
     std::mutex mu;
     std::condition_variable cv_done, cv_work;
+    // A ticket's record: its timestamps and, until taken, its result.  The
+    // record is dropped once the result has been taken and the timestamps
+    // read; records of taken results whose timestamps are never read are
+    // dropped oldest first beyond kTakenKept, so a long-running server holds
+    // host memory only for tickets in flight or not yet taken.
     struct Item {
         double t[3] = {0, 0, 0};  // submit, admit, done (seconds on the server clock)
-        bool done = false;
-        Result res;
+        bool done = false, taken = false, timed = false;
+        std::unique_ptr<Result> res;
     };
+    static constexpr size_t kTakenKept = 1u << 20;
     std::map<uint64_t, Item> items;
+    std::deque<uint64_t> taken_untimed;  // taken tickets whose timestamps were not read yet
     std::deque<std::unique_ptr<BatchJob>> incoming;  // submitted, not yet seen by the worker
     uint64_t next = 1;
     bool stop = false;
@@ -45,7 +52,7 @@ struct Server {
         {
             std::lock_guard<std::mutex> lk(mu);
             auto &it = items[id];
-            it.res = std::move(r);
+            it.res.reset(new Result(std::move(r)));
             it.t[2] = now();
             it.done = true;
         }
@@ -53,17 +60,16 @@ struct Server {
     }
 };
 
-static void fail_job(Server *s, uint64_t id, gfsm_status code)
-{
-    Result r;
-    r.status = code;
-    s->finish(id, r);
-}
-
 static void worker_main(Server *s)
 {
     std::deque<std::unique_ptr<BatchJob>> pending;
-    auto done = [s](std::unique_ptr<BatchJob> job, Result &r) { s->finish(job->id, r); };
+    // A job's query stays pinned until its result is recorded.
+    auto done = [s](std::unique_ptr<BatchJob> job, Result &r) {
+        const HostGraph *q = job->query;
+        s->finish(job->id, r);
+        job.reset();
+        unpin_graph(*q);
+    };
     for (;;) {
         {
             std::unique_lock<std::mutex> lk(s->mu);
@@ -88,16 +94,18 @@ static void worker_main(Server *s)
                 Result r;
                 Status st = match_plain(*s->ctx, *s->data, *job.query, job.opts, r);
                 if (!st.good() && st.code != GFSM_ERR_CAP) r.status = st.code;
+                done(std::move(pending.front()), r);
                 pending.pop_front();
-                s->finish(id, r);
                 continue;
             }
             Status st;
             const double t = s->now();
             if (!s->eng->try_admit(pending.front(), st)) {
                 if (!st.good()) {  // can never fit
+                    Result r;
+                    r.status = st.code;
+                    done(std::move(pending.front()), r);
                     pending.pop_front();
-                    fail_job(s, id, st.code);
                     continue;
                 }
                 break;
@@ -117,14 +125,42 @@ static void worker_main(Server *s)
         }
     }
     // Device error: fail everything still queued.
-    for (auto &j : pending) fail_job(s, j->id, GFSM_ERR_DEVICE);
+    for (auto &j : pending) {
+        Result r;
+        r.status = GFSM_ERR_DEVICE;
+        done(std::move(j), r);
+    }
     std::lock_guard<std::mutex> lk(s->mu);
     for (auto &kv : s->items)
         if (!kv.second.done) {
-            kv.second.res.status = GFSM_ERR_DEVICE;
+            kv.second.res.reset(new Result);
+            kv.second.res->status = GFSM_ERR_DEVICE;
             kv.second.done = true;
         }
     s->cv_done.notify_all();
+}
+
+// Stop the worker once the submitted work is done, release the engine and
+// the data graph, and leave the context; the Server object stays until
+// server_stop (its finished results can still be taken).
+static void server_halt(void *p)
+{
+    Server *s = static_cast<Server *>(p);
+    if (!s->ctx) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    if (s->worker.joinable()) s->worker.join();
+    s->eng.reset();
+    {
+        std::lock_guard<std::mutex> lk(s->ctx->srv_mu);
+        s->ctx->servers.erase(s);
+    }
+    unpin_graph(*s->data);
+    s->ctx = nullptr;
+    s->data = nullptr;
 }
 
 Status server_start(Ctx &ctx, const HostGraph &data, const gfsm_serve_opts &opts, Server *&out)
@@ -150,6 +186,12 @@ Status server_start(Ctx &ctx, const HostGraph &data, const gfsm_serve_opts &opts
     // One query may take up to a sixteenth of the budget (its projection
     // decides how much below that).
     s->eng->set_max_region(std::max(s->eng->min_region(GFSM_MAXQ), budget / 16));
+    pin_graph(data);
+    {
+        std::lock_guard<std::mutex> lk(ctx.srv_mu);
+        ctx.servers.insert(s.get());
+        ctx.halt_server = server_halt;
+    }
     s->worker = std::thread(worker_main, s.get());
     out = s.release();
     return Status::ok();
@@ -157,6 +199,10 @@ Status server_start(Ctx &ctx, const HostGraph &data, const gfsm_serve_opts &opts
 
 Status server_submit(Server &s, const HostGraph &query, const gfsm_match_opts *opts, uint64_t &ticket)
 {
+    {
+        std::lock_guard<std::mutex> lk(s.mu);
+        if (s.stop || !s.data) return s.fatal.good() ? Status::err(GFSM_ERR_ARG, "server stopped") : s.fatal;
+    }
     std::unique_ptr<BatchJob> job(new BatchJob);
     job->query = &query;
     if (opts) job->opts = *opts;
@@ -171,10 +217,12 @@ Status server_submit(Server &s, const HostGraph &query, const gfsm_match_opts *o
         auto &it = s.items[ticket];
         it.t[0] = s.now();
         if (!plan_st.good()) {
-            it.res.status = plan_st.code;
+            it.res.reset(new Result);
+            it.res->status = plan_st.code;
             it.t[1] = it.t[2] = it.t[0];
             it.done = true;
         } else {
+            pin_graph(query);  // released when its result is recorded
             s.incoming.push_back(std::move(job));
         }
     }
@@ -187,11 +235,25 @@ Status server_take(Server &s, uint64_t ticket, bool wait, Result &out)
     std::unique_lock<std::mutex> lk(s.mu);
     auto it = s.items.find(ticket);
     if (it == s.items.end()) return Status::err(GFSM_ERR_ARG, "unknown ticket");
+    if (it->second.taken) return Status::err(GFSM_ERR_ARG, "the ticket's result was already taken");
     if (!it->second.done) {
         if (!wait) return Status::err(GFSM_ERR_PENDING, "pending");
         s.cv_done.wait(lk, [&] { return it->second.done; });
     }
-    out = std::move(it->second.res);
+    Server::Item &item = it->second;
+    out = item.res ? std::move(*item.res) : Result{};
+    if (!item.res) out.status = GFSM_ERR_DEVICE;
+    item.res.reset();
+    item.taken = true;
+    if (item.timed) {
+        s.items.erase(it);
+    } else {
+        s.taken_untimed.push_back(ticket);
+        while (s.taken_untimed.size() > Server::kTakenKept) {
+            s.items.erase(s.taken_untimed.front());
+            s.taken_untimed.pop_front();
+        }
+    }
     return Status::ok();
 }
 
@@ -199,19 +261,16 @@ Status server_times(Server &s, uint64_t ticket, double t[3])
 {
     std::lock_guard<std::mutex> lk(s.mu);
     auto it = s.items.find(ticket);
-    if (it == s.items.end()) return Status::err(GFSM_ERR_ARG, "unknown ticket");
+    if (it == s.items.end()) return Status::err(GFSM_ERR_ARG, "unknown ticket (or its record was dropped)");
     for (int i = 0; i < 3; i++) t[i] = it->second.t[i];
+    it->second.timed = true;
+    if (it->second.taken) s.items.erase(it);  // its entry in taken_untimed is skipped when evicted
     return Status::ok();
 }
 
 void server_stop(Server *s)
 {
-    {
-        std::lock_guard<std::mutex> lk(s->mu);
-        s->stop = true;
-    }
-    s->cv_work.notify_all();
-    if (s->worker.joinable()) s->worker.join();
+    server_halt(s);
     delete s;
 }
 

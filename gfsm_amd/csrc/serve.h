@@ -72,7 +72,9 @@ public:
     double calibration() const { return calib_; }
     uint64_t min_region(uint32_t q) const;
     uint64_t max_region() const { return max_region_; }
-    void set_max_region(uint64_t b) { max_region_ = b; }
+    // Largest region a query may take, rounded down to the region alignment
+    // so that every region offset keeps it.
+    void set_max_region(uint64_t b);
 
     // Admit a planned job into a free slot when its region fits; false when
     // no slot or no memory is free right now.  The job's region is taken

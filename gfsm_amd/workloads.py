@@ -65,6 +65,37 @@ def config2_windows(data, queries, ks=CONFIG2_ROOT_K):
     return out
 
 
+# The tail windows above cover only the low-degree end of each query's roots.
+# The timed sample adds, per query, two windows of CONFIG2_STRATUM_ROOTS roots
+# from the rest of the degree range: the lowest-id roots (the R-MAT hubs) and
+# the roots from the median root on.  Their subtrees are far beyond
+# enumeration (one hub root yields > 10^9 candidates by depth 7), so they run
+# under the deterministic per-depth cap CONFIG2_STRATUM_CAP: every depth
+# tests only its first that many candidates in enumeration order (roots by
+# id, then segment order), and the partial count is the same on every path
+# and in the oracle's canonical enumeration.
+CONFIG2_STRATUM_ROOTS = 64
+CONFIG2_STRATUM_CAP = 1 << 25
+CONFIG2_STRATA = ("hub", "middle", "tail")
+
+
+def config2_sample(data, queries, ks=CONFIG2_ROOT_K):
+    """The timed sample of the power-law workload: for each query, a hub-end,
+    a middle and a tail-end root window.  Returns [(query index, stratum,
+    root_lo, root_hi, max_depth_cand)], max_depth_cand 0 = uncapped."""
+    out = []
+    tails = config2_windows(data, queries, ks)
+    for i, q in enumerate(queries):
+        roots = data.roots(q)
+        n = len(roots)
+        k = min(CONFIG2_STRATUM_ROOTS, n)
+        out.append((i, "hub", int(roots[0]), int(roots[k - 1]) + 1, CONFIG2_STRATUM_CAP))
+        m = n // 2
+        out.append((i, "middle", int(roots[m]), int(roots[min(m + k, n) - 1]) + 1, CONFIG2_STRATUM_CAP))
+        out.append((i, "tail", tails[i][0], tails[i][1], 0))
+    return out
+
+
 def config3_data(side=1400):
     return Graph.grid(side, side, del_frac=0.05, diag_frac=0.02, seed=3000, mode=MODE_REF)
 
@@ -179,9 +210,14 @@ def config5_queries(data, n=10000):
 # many candidates in enumeration order); a query that reaches it reports a
 # partial (capped) count that is the same served and serial.  CONFIG5_CAP
 # bounds all candidates of a query above q x CONFIG5_DEPTH_CAP (never reached).
-CONFIG5_FIRST = 256
-CONFIG5_DEPTH_CAP = 1 << 20
-CONFIG5_CAP = 1 << 28
+# Sized so that most queries finish: measured on the first 4,000 queries,
+# 77.5% finish (with 256 depth-1 candidates and a 2^20 cap, 47% did).
+CONFIG5_FIRST = 16
+CONFIG5_DEPTH_CAP = 1 << 22
+CONFIG5_CAP = 1 << 30
+# Every CONFIG5_FIXTURE_STRIDE-th query of the stream has committed oracle
+# results (tests/golden/fixtures_config5.json).
+CONFIG5_FIXTURE_STRIDE = 16
 
 
 def config5_windows(data, queries, max_first=CONFIG5_FIRST):

@@ -231,6 +231,13 @@ typedef struct gfsm_serve_opts {
     uint32_t flags;         /* GFSM_DIGEST etc., applied to every query                        */
 } gfsm_serve_opts;
 
+/* Lifetimes: the server pins `data` and every query in flight.
+ * gfsm_graph_free on a pinned graph only marks it; the graph is freed when
+ * the server lets go of it (the query's result is recorded, or the server
+ * stops).  gfsm_close(ctx) first stops every server still running on ctx:
+ * each finishes the work already submitted, its finished results can still
+ * be taken, and later submits fail with GFSM_ERR_ARG; gfsm_server_stop must
+ * still be called to free the server. */
 gfsm_status gfsm_server_start(gfsm_ctx *ctx, const gfsm_graph *data, const gfsm_serve_opts *opts,
                               gfsm_server **srv);
 /* opts (may be NULL): this query's flags (added to the server's), root
@@ -238,10 +245,15 @@ gfsm_status gfsm_server_start(gfsm_ctx *ctx, const gfsm_graph *data, const gfsm_
  * each query's memory by projection). */
 gfsm_status gfsm_server_submit(gfsm_server *srv, const gfsm_graph *query, const gfsm_match_opts *opts,
                                uint64_t *ticket);
-/* GFSM_ERR_PENDING while running; on GFSM_OK *r is owned by the caller. */
+/* GFSM_ERR_PENDING while running; on GFSM_OK *r is owned by the caller.  A
+ * ticket's result is handed out once: a second poll or wait on it returns
+ * GFSM_ERR_ARG. */
 gfsm_status gfsm_server_poll(gfsm_server *srv, uint64_t ticket, gfsm_result **r);
 gfsm_status gfsm_server_wait(gfsm_server *srv, uint64_t ticket, gfsm_result **r);
-/* Per-ticket timestamps (seconds on the server clock): submit, admit, done. */
+/* Per-ticket timestamps (seconds on the server clock): submit, admit, done.
+ * A ticket's record is dropped once its result has been taken and its
+ * timestamps read (then GFSM_ERR_ARG); of taken results whose timestamps are
+ * never read, the server keeps the most recent 2^20 records. */
 gfsm_status gfsm_server_times(gfsm_server *srv, uint64_t ticket, double t[3]);
 void        gfsm_server_stop(gfsm_server *srv);
 

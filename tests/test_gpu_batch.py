@@ -198,3 +198,41 @@ def test_deterministic_last_cap_same_on_every_path(ctx):
         finally:
             srv.stop()
         assert [(x.count, x.digest, x.status) for x in got] == [(x.count, x.digest, x.status) for x in a], M
+
+
+@pytest.mark.gpu
+def test_server_ticket_lifetime_and_graph_pins(er_vlabel):
+    """A result is handed out once (a second poll or wait fails with
+    GFSM_ERR_ARG); a ticket's record is dropped once its result is taken and
+    its timestamps read; a query graph freed while in flight, and the data
+    graph freed while the server runs, stay valid until the server lets go;
+    closing the context stops its servers after their submitted work."""
+    c = G.Context(0)
+    data = G.Graph.er(1500, 8.0, nvlabels=3, nelabels=1, seed=77, mode=G.MODE_VLABEL)
+    qs = _queries(data, [4, 5, 3, 6], seed=6100)
+    alone = [c.match(data, q, digest=True) for q in qs]
+    srv = G.Server(c, data, max_inflight=2, digest=True)
+    tk = [srv.submit(q) for q in qs]
+    for q in qs:  # the server holds its own pins on queries in flight
+        q.free()
+    data.free()
+    first = srv.wait(tk[0])
+    assert (first.count, first.digest) == (alone[0].count, alone[0].digest)
+    with pytest.raises(G.GfsmError) as ei:
+        srv.wait(tk[0])
+    assert ei.value.code == 2
+    t0, t1, t2 = srv.times(tk[0])
+    assert t0 <= t1 <= t2
+    with pytest.raises(G.GfsmError) as ei:  # taken and timed: the record is gone
+        srv.times(tk[0])
+    assert ei.value.code == 2
+    c.close()  # stops the server after the submitted work
+    rest = [srv.wait(t) for t in tk[1:]]
+    assert [(m.count, m.digest) for m in rest] == [(a.count, a.digest) for a in alone[1:]]
+    with pytest.raises(G.GfsmError) as ei:
+        srv.poll(tk[1])
+    assert ei.value.code == 2
+    with pytest.raises(G.GfsmError) as ei:
+        srv.submit(G.Graph.walk_query(er_vlabel, 3, seed=1))
+    assert ei.value.code == 2
+    srv.stop()

@@ -212,6 +212,10 @@ def test_binary_image_rejects_damage(tmp_path):
         "payload_bit": raw[:header + 40] + bytes([raw[header + 40] ^ 1]) + raw[header + 41:],
         "last_byte": raw[:-1] + bytes([raw[-1] ^ 0x80]),
         "short_header": raw[:20],
+        # a degree beyond the kernels' 2^26 bound, and counts whose byte sizes
+        # would wrap a 64-bit sum: rejected before anything is allocated
+        "max_degree": raw[:20] + (1 << 26).to_bytes(4, "little") + raw[24:],
+        "huge_counts": raw[:24] + (1 << 62).to_bytes(8, "little") + raw[32:40] + ((1 << 64) - (1 << 62)).to_bytes(8, "little") + raw[48:],
     }
     for name, blob in cases.items():
         q = tmp_path / f"{name}.bin"

@@ -37,6 +37,8 @@ WALK_DENSE = 0
 WALK_SPARSE = 1
 ERR_PENDING = 6
 MAX_Q = 32
+NOALLOC_SCRATCH_BYTES = 64 << 20  # GFSM_NOALLOC_SCRATCH_BYTES
+NOALLOC_LEVEL_BYTES = 60 << 20    # its level part (the rest holds the scan workspace)
 
 
 class GfsmError(RuntimeError):
@@ -148,6 +150,7 @@ def lib():
         "gfsm_query_projection": [vp, vp, P(MatchOpts), P(C.c_double), P(C.c_double)],
         "gfsm_query_window": [vp, vp, u64, pu32, pu32, P(u64)],
         "gfsm_match": [vp, vp, vp, P(MatchOpts), P(vp)],
+        "gfsm_match_noalloc": [vp, vp, vp, P(MatchOpts), P(u64), P(u64)],
         "gfsm_result_count": [vp, P(u64)],
         "gfsm_result_rows": [vp, P(pu32), P(u32), P(u64)],
         "gfsm_result_digest": [vp, P(u64)],
@@ -447,6 +450,18 @@ class Context:
             del m
         _check(rc, "match")
         return Match(r)
+
+    def match_noalloc(self, data: Graph, query: Graph, **kw):
+        """gfsm_match_noalloc: count (and digest) without allocating device
+        memory; data must have been uploaded.  Returns (status, count,
+        digest): status 0, or 5 with the partial values under a cap."""
+        o = _opts(**kw)
+        n = C.c_uint64()
+        dg = (C.c_uint64 * 2)()
+        rc = lib().gfsm_match_noalloc(self._h, data._h, query._h, C.byref(o), C.byref(n), dg)
+        if rc not in (0, 5):
+            _check(rc, "match_noalloc")
+        return rc, n.value, (int(dg[0]), int(dg[1]))
 
     def match_batch(self, data: Graph, queries, windows=None, caps=None, depth_caps=None, **kw):
         """Batched matching: every query of the list in flight together.

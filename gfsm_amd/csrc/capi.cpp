@@ -311,6 +311,25 @@ gfsm_status gfsm_match(gfsm_ctx *ctx, const gfsm_graph *data, const gfsm_graph *
     GUARD_END
 }
 
+gfsm_status gfsm_match_noalloc(gfsm_ctx *ctx, const gfsm_graph *data, const gfsm_graph *query,
+                               const gfsm_match_opts *opts, uint64_t *count, uint64_t digest[2])
+{
+    static_assert(GFSM_NOALLOC_SCRATCH_BYTES == kNoAllocScratchBytes, "scratch size in gfsm.h");
+    GUARD_BEGIN
+    if (!ctx || !data || !query || !count || !digest) return fail(Status::err(GFSM_ERR_ARG, "null argument"));
+    gfsm_match_opts o{};
+    if (opts) o = *opts;
+    Result res;
+    Status st = match_noalloc(*ctx->c, *data->g, *query->g, o, res);
+    if (st.good() || (st.code == GFSM_ERR_CAP && res.status == GFSM_ERR_CAP)) {
+        *count = res.count;
+        digest[0] = res.digest[0];
+        digest[1] = res.digest[1];
+    }
+    return st.good() ? ok() : fail(st);
+    GUARD_END
+}
+
 gfsm_status gfsm_match_write_fs(gfsm_ctx *ctx, const gfsm_graph *data, const gfsm_graph *query,
                                 const gfsm_match_opts *opts, const char *path, gfsm_result **r)
 {

@@ -107,6 +107,13 @@ Status ctx_open(int device, uint64_t budget, Ctx *&out)
         c->lane.destroy();
         return st;
     }
+    // The allocation-free match's device array: its address on this device
+    // (looked up once per device; the array lives until the process ends).
+    hipError_t se = noalloc_scratch(device, c->lane.stream, &c->scratch);
+    if (se != hipSuccess) {
+        c->lane.destroy();
+        return hip_status(se, "device scratch of the allocation-free match");
+    }
     out = c.release();
     return Status::ok();
 }
@@ -165,6 +172,12 @@ void release_graph_mirrors(const HostGraph &g)
         free_mirror(kv.second);
     }
     g.dev.clear();
+}
+
+std::mutex &noalloc_mutex(int device)
+{
+    static std::mutex mu[64];
+    return mu[(unsigned)device % 64];
 }
 
 static std::mutex g_pin_mu;  // guards HostGraph::pins and free_pending
@@ -950,6 +963,61 @@ Status match_plain(Ctx &ctx, const HostGraph &data, const HostGraph &query, cons
     st = upload_graph(ctx, data, dg);
     if (!st.good()) return st;
     st = run_match(ctx, data, *dg, plan, o, res, stream, ctx.lane, nullptr);
+    res.stats.host_ms = now_ms() - t0;
+    return st;
+}
+
+Status match_noalloc(Ctx &ctx, const HostGraph &data, const HostGraph &query, const gfsm_match_opts &o, Result &res)
+{
+    const double t0 = now_ms();
+    res = Result{};
+    if (o.flags & ~(GFSM_DIGEST | GFSM_ORDER_COST))
+        return Status::err(GFSM_ERR_ARG, "the allocation-free match counts only: flags may hold GFSM_DIGEST and "
+                                         "GFSM_ORDER_COST");
+    if (o.mem_budget && o.mem_budget > kNoAllocLevelBytes)
+        return Status::err(GFSM_ERR_ARG, "mem_budget exceeds the allocation-free match's level scratch (" +
+                                             std::to_string(kNoAllocLevelBytes) + " bytes)");
+    if (data.mode != query.mode) return Status::err(GFSM_ERR_ARG, "query and data graph use different modes");
+    Plan plan;
+    Status st = plan_for(data, query, o, plan);
+    if (!st.good()) return st;
+    res.width = plan.q;
+    res.stats.q = plan.q;
+    if (plan.q == 1) {
+        st = match_single_vertex(data, plan, o, res);
+        res.stats.host_ms = now_ms() - t0;
+        return st;
+    }
+    DevGraph *dg = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_dev_mu);
+        auto it = data.dev.find(&ctx);
+        if (it != data.dev.end()) dg = it->second;
+    }
+    if (!dg) return Status::err(GFSM_ERR_ARG, "the data graph is not resident on this context (gfsm_graph_upload)");
+    if (!ctx.scratch) return Status::err(GFSM_ERR_DEVICE, "no device scratch on this context");
+    // Calls on one device share its scratch array: one at a time.
+    std::lock_guard<std::mutex> dev_lk(noalloc_mutex(ctx.device));
+    std::lock_guard<std::mutex> lk(ctx.mu);
+    HIP_TRY(hipSetDevice(ctx.device));
+    // The lane: this context's stream, events and pinned totals; the scan
+    // workspace, device totals, tile index and levels in the scratch array.
+    char *p = ctx.scratch;
+    Lane L;
+    L.stream = ctx.lane.stream;
+    L.ev[0] = ctx.lane.ev[0];
+    L.ev[1] = ctx.lane.ev[1];
+    L.tot_h = ctx.lane.tot_h;
+    L.ws.ctl = reinterpret_cast<uint32_t *>(p);
+    L.tot_d = reinterpret_cast<Totals *>(p + 256);
+    L.ws.wk = reinterpret_cast<uint64_t *>(p + 512);
+    L.ws.wc = L.ws.wk + kNoAllocTiles;
+    L.tile_node = L.ws.wc + kNoAllocTiles;
+    L.ws.max_tiles = kNoAllocTiles;
+    Arena levels;
+    levels.base = p + kNoAllocWorkBytes;
+    levels.bytes = kNoAllocLevelBytes;
+    st = run_match(ctx, data, *dg, plan, o, res, nullptr, L, &levels);  // L borrows: nothing to release
     res.stats.host_ms = now_ms() - t0;
     return st;
 }

@@ -89,6 +89,7 @@ struct Ctx {
     Arena arena;
     Arena batch_arena;  // level regions of the batched path (grow-only, shared by its engines)
     Tuning tune;
+    char *scratch = nullptr;  // this device's allocation-free match array (kernels.hip), not owned
     std::mutex mu;
     std::set<const HostGraph *> graphs;  // data graphs resident on this context
     // Batched-matching engine kept between calls (its arena, slots and
@@ -127,6 +128,14 @@ void free_graph_when_unpinned(HostGraph *g);
 // header the caller wrote) instead of keeping them; unsorted only.
 Status match_plain(Ctx &ctx, const HostGraph &data, const HostGraph &query, const gfsm_match_opts &opts,
                    Result &res, FILE *stream = nullptr);
+
+// gfsm_match in count mode without allocating: the data graph must already
+// be resident on ctx; levels, scan workspace and totals live in the device's
+// scratch array (kNoAllocScratchBytes); only dense tables built earlier are
+// used.  Calls on one device are serialised (noalloc_mutex).
+Status match_noalloc(Ctx &ctx, const HostGraph &data, const HostGraph &query, const gfsm_match_opts &opts,
+                     Result &res);
+std::mutex &noalloc_mutex(int device);
 
 // The query's plan for these options: the reference order, or with
 // GFSM_ORDER_COST the data-statistics order (root windows then rejected).

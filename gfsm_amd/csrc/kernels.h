@@ -264,6 +264,21 @@ constexpr uint32_t kScanTestItems = 2;
 hipError_t launch_seg_hash_build(const uint32_t *dir_ptr, const uint32_t *dir, uint32_t V, uint4 *slots,
                                  uint32_t mask, hipStream_t s);
 
+// Device scratch of the allocation-free match: a file-scope array of
+// kNoAllocScratchBytes in kernels.hip, one per GPU, held until the process
+// ends.  Layout: control block (256 B), totals (256 B), the scan words of
+// kNoAllocTiles tiles (two arrays), the tile index (2 words per tile), then
+// from kNoAllocWorkBytes on the level arena of kNoAllocLevelBytes.
+constexpr uint64_t kNoAllocScratchBytes = 64ull << 20;
+constexpr uint64_t kNoAllocTiles = 1ull << 16;
+constexpr uint64_t kNoAllocWorkBytes = 4ull << 20;
+constexpr uint64_t kNoAllocLevelBytes = kNoAllocScratchBytes - kNoAllocWorkBytes;
+static_assert(512 + 32 * kNoAllocTiles <= kNoAllocWorkBytes, "scratch workspace layout");
+// Device address of `device`'s scratch array: looked up with
+// hipGetSymbolAddress on the first call for a device (that device made
+// current; its workspace part cleared on `s`), then cached.
+hipError_t noalloc_scratch(int device, hipStream_t s, char **ptr);
+
 // Zero the control block of a workspace (device kernel; safe under capture).
 hipError_t launch_ws_reset(ScanWS ws, Totals *tot, hipStream_t s);
 // Zero every tile word (needed once per 2^kEpochBits launches on a lane).

@@ -26,6 +26,8 @@
 // persistent step kernel takes (query, tile) work items by ticket.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "kernels.h"
 
 namespace gfsm {
@@ -1503,6 +1505,47 @@ hipError_t launch_seg_hash_build(const uint32_t *dir_ptr, const uint32_t *dir, u
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_seg_insert, dim3(4096), dim3(256), 0, s, dir_ptr, dir, V, slots, mask);
     return hipGetLastError();
+}
+
+// The allocation-free match's scratch.  In host code `scratch` names a
+// host-side stand-in, not device memory: every use goes through the address
+// hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+hipError_t noalloc_scratch(int device, hipStream_t s, char **ptr)
+{
+    static std::mutex mu;
+    static char *addr[64] = {};
+    *ptr = nullptr;
+    if (device < 0 || device >= 64) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!addr[device]) {
+        int cur = 0;
+        hipError_t e = hipGetDevice(&cur);
+        if (e == hipSuccess) e = hipSetDevice(device);
+        if (e != hipSuccess) return e;
+        char *p = nullptr;
+        e = hipGetSymbolAddress(reinterpret_cast<void **>(&p), HIP_SYMBOL(scratch));
+        if (e == hipSuccess && !p) e = hipErrorInvalidSymbol;
+        if (e == hipSuccess) {
+            // Zero the workspace words once: the control block, totals, tile
+            // words and tile index (each call clears what it uses again).
+            ScanWS ws{reinterpret_cast<uint64_t *>(p + 512), reinterpret_cast<uint64_t *>(p + 512) + kNoAllocTiles,
+                      reinterpret_cast<uint32_t *>(p), kNoAllocTiles};
+            hipLaunchKernelGGL(k_ws_clear, dim3(1024), dim3(256), 0, s, ws);
+            e = hipGetLastError();
+            if (e == hipSuccess) {
+                hipLaunchKernelGGL(k_ws_reset, dim3(1), dim3(64), 0, s, ws, reinterpret_cast<Totals *>(p + 256));
+                e = hipGetLastError();
+            }
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+        }
+        (void)hipSetDevice(cur);
+        if (e != hipSuccess) return e;
+        addr[device] = p;
+    }
+    *ptr = addr[device];
+    return hipSuccess;
 }
 
 hipError_t launch_ws_reset(ScanWS ws, Totals *tot, hipStream_t s)

@@ -207,6 +207,24 @@ gfsm_status gfsm_match_write_fs(gfsm_ctx *ctx, const gfsm_graph *data, const gfs
                                 const gfsm_match_opts *opts, const char *path, gfsm_result **r);
 void        gfsm_result_free(gfsm_result *r);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc: gfsm_match in count mode (opts->flags: GFSM_DIGEST,
+ * GFSM_ORDER_COST) that allocates nothing.  Its levels, scan workspace and
+ * device totals live in a library-owned device array of
+ * GFSM_NOALLOC_SCRATCH_BYTES per GPU (64 MiB, held until the process ends;
+ * 60 MiB of it for levels): chunked expansion keeps any query inside it, and
+ * opts->mem_budget above the level part is rejected (GFSM_ERR_ARG).  `data`
+ * must already be resident on ctx (gfsm_graph_upload); dense segment tables
+ * are used only where an earlier gfsm_match built them.  Writes the count
+ * and digest sums; GFSM_ERR_CAP with the partial values when max_cand or
+ * max_depth_cand stops it.  All calls on one GPU share its array: the
+ * library runs them one at a time (they would race on it otherwise), and
+ * each clears the words it needs. */
+#define GFSM_NOALLOC_SCRATCH_BYTES (64ull << 20)
+gfsm_status gfsm_match_noalloc(gfsm_ctx *ctx, const gfsm_graph *data, const gfsm_graph *query,
+                               const gfsm_match_opts *opts, uint64_t *count, uint64_t digest[2]);
+
 /* Row digest used by GFSM_DIGEST (per seed: mix(sum over columns c of
  * mix(seed ^ (row[c] | c << 32))), mix = the SplitMix64 finaliser): h = sum over rows of two seeded hashes of
  * the row (column order = original query id). */

@@ -51,25 +51,22 @@ def parse():
     p.add_argument("--sections", default="plain,grid,batch,serve",
                    help="comma-separated sections to run (plain is the headline value)")
     p.add_argument("--serve-queries", type=int, default=10000, help="length of the served stream")
-    p.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
-                   help="only run an elementwise int32 kernel that reads and writes MIB MiB, 5 times "
-                        "(scales the HBM byte counters of a profiler run)")
+    p.add_argument("--calibrate", choices=("stream", "gather", "gather2"), default=None,
+                   help="only run libgfsm's calibration kernel of known HBM footprint, 5 launches "
+                        "(scales the byte counters of a profiler run): stream = 1 GiB read and written, "
+                        "gather = one word from each of 2^24 distinct 128-byte lines (2 GiB of lines), "
+                        "gather2 = both 64-byte halves of the same lines")
     return p.parse_args()
 
 
-def calibrate_copy(mib):
-    """A kernel of known HBM traffic: out = in + 1 over mib MiB of int32,
-    i.e. mib MiB read and mib MiB written per launch."""
-    import torch
+def calibrate(mode):
+    """libgfsm's calibration kernel of known HBM footprint, 5 launches."""
+    import gfsm_amd as G
 
-    n = mib * (1 << 20) // 4
-    src = torch.zeros(n, dtype=torch.int32, device="cuda")
-    dst = torch.empty_like(src)
-    for _ in range(5):
-        torch.add(src, 1, out=dst)
-    torch.cuda.synchronize()
-    print(json.dumps({"calibrate_copy_mib": mib, "launches": 5, "read_bytes_per_launch": n * 4,
-                      "write_bytes_per_launch": n * 4}))
+    with G.Context(0) as ctx:
+        n = (1 << 28) if mode == "stream" else (1 << 24)
+        rd, wr = ctx.selftest_traffic({"stream": 0, "gather": 1, "gather2": 2}[mode], n, 5)
+    print(json.dumps({"calibrate": mode, "launches": 5, "read_bytes_per_launch": rd, "write_bytes_per_launch": wr}))
 
 
 def traffic_from_profiles():
@@ -357,8 +354,8 @@ def grid_section(steps):
 
 def main():
     args = parse()
-    if args.calibrate_copy:
-        calibrate_copy(args.calibrate_copy)
+    if args.calibrate:
+        calibrate(args.calibrate)
         return
     if args.gpus != 1:
         print(f"bench.py: the matcher runs on one GPU; --gpus {args.gpus} is not supported "

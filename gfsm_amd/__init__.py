@@ -171,6 +171,7 @@ def lib():
         "gfsm_server_times": [vp, u64, P(C.c_double)],
         "gfsm_server_stop": [vp],
         "gfsm_selftest_scan": [vp, pu32, u64, P(u64), pu32, P(u64)],
+        "gfsm_selftest_traffic": [vp, u32, u64, u32, P(u64)],
         "gfsm_gen_er": [u32, C.c_double, u32, u32, u64, u32, P(vp)],
         "gfsm_gen_rmat": [u32, u32, C.c_double, C.c_double, C.c_double, u32, C.c_double, u32, u64, u32, P(vp)],
         "gfsm_gen_grid": [u32, u32, C.c_double, C.c_double, u64, u32, P(vp)],
@@ -411,6 +412,13 @@ class Context:
 
     def __exit__(self, *a):
         self.close()
+
+    def selftest_traffic(self, mode, n, launches=5):
+        """Calibration launches of known HBM footprint (mode 0 stream, 1
+        gather); returns (bytes read, bytes written) per launch."""
+        b = (C.c_uint64 * 2)()
+        _check(lib().gfsm_selftest_traffic(self._h, mode, n, launches, b), "selftest_traffic")
+        return int(b[0]), int(b[1])
 
     def selftest_scan(self, vals):
         """Chained wave64 scan on the device: (exclusive prefix u64, compacted indices, (nonzero, sum))."""

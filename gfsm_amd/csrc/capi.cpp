@@ -601,6 +601,15 @@ gfsm_status gfsm_selftest_scan(gfsm_ctx *ctx, const uint32_t *vals, uint64_t n, 
     GUARD_END
 }
 
+gfsm_status gfsm_selftest_traffic(gfsm_ctx *ctx, uint32_t mode, uint64_t n, uint32_t launches, uint64_t bytes[2])
+{
+    GUARD_BEGIN
+    if (!ctx || !bytes) return fail(Status::err(GFSM_ERR_ARG, "null argument"));
+    Status st = selftest_traffic(*ctx->c, mode, n, launches, bytes);
+    return st.good() ? ok() : fail(st);
+    GUARD_END
+}
+
 #define GEN_RETURN(call)                           \
     GUARD_BEGIN                                    \
     if (!g) return fail(Status::err(GFSM_ERR_ARG, "null argument")); \

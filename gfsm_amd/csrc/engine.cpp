@@ -875,8 +875,11 @@ Status run_match(Ctx &ctx, const HostGraph &data, const DevGraph &dg, const Plan
     const uint64_t budget = o.mem_budget ? o.mem_budget : ctx.budget;
     Status st = run.carve(fixed ? std::min(budget, fixed->bytes) : budget);
     if (!st.good()) return st;
-    if (o.max_depth_cand)  // capped: start with an eighth of the cap, then follow the observed fan-out
-        for (uint32_t k = 0; k < GFSM_MAXQ; k++)
+    // Capped: inner depths start with an eighth of the cap, then follow the
+    // observed fan-out.  The counting last depth keeps whole launches: it
+    // needs no level buffer, and its ranges are trimmed to the cap exactly.
+    if (o.max_depth_cand)
+        for (uint32_t k = 0; k + 1 < plan.q; k++)
             run.chunk[k] = std::max<uint64_t>(2048, std::min<uint64_t>(run.chunk[k], o.max_depth_cand / kCapStartDiv));
     uint64_t rlo, rhi;
     root_range(data, plan, o.root_lo, o.root_hi, rlo, rhi);
@@ -940,6 +943,28 @@ Status run_match(Ctx &ctx, const HostGraph &data, const DevGraph &dg, const Plan
 }
 
 }  // namespace
+
+Status selftest_traffic(Ctx &ctx, uint32_t mode, uint64_t n, uint32_t launches, uint64_t bytes[2])
+{
+    if (mode > 2 || n == 0 || (mode >= 1 && (n & (n - 1))) || launches == 0)
+        return Status::err(GFSM_ERR_ARG, "mode 0 (stream), 1 or 2 (gather, n a power of two), n > 0, launches > 0");
+    std::lock_guard<std::mutex> lk(ctx.mu);
+    HIP_TRY(hipSetDevice(ctx.device));
+    const uint64_t src_words = mode ? 32 * n : n;
+    const uint64_t dst_words = mode == 2 ? 2 * n : n;
+    uint32_t *src = nullptr, *dst = nullptr;
+    hipError_t e = hipMalloc(&src, src_words * 4);
+    if (e == hipSuccess) e = hipMalloc(&dst, dst_words * 4);
+    if (e == hipSuccess) e = hipMemsetAsync(src, 0, src_words * 4, ctx.lane.stream);
+    for (uint32_t k = 0; k < launches && e == hipSuccess; k++) e = launch_calibrate(mode, src, dst, n, ctx.lane.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx.lane.stream);
+    (void)hipFree(src);
+    (void)hipFree(dst);
+    if (e != hipSuccess) return hip_status(e, "selftest_traffic");
+    bytes[0] = mode ? 128 * n : 4 * n;  // read per launch: whole lines for the gathers
+    bytes[1] = 4 * dst_words;           // written per launch
+    return Status::ok();
+}
 
 Status match_plain(Ctx &ctx, const HostGraph &data, const HostGraph &query, const gfsm_match_opts &o, Result &res,
                    FILE *stream)

@@ -148,6 +148,9 @@ void root_range(const HostGraph &data, const Plan &plan, uint32_t root_lo, uint3
 Status selftest_scan(Ctx &ctx, const uint32_t *vals, uint64_t n, uint64_t *prefix, uint32_t *idx,
                      uint64_t totals[2]);
 uint64_t alg_bytes(const Plan &plan, const gfsm_stats &st, uint32_t flags);
+// Runs `launches` calibration launches (launch_calibrate) over n words;
+// bytes = {HBM bytes read, written} per launch by construction.
+Status selftest_traffic(Ctx &ctx, uint32_t mode, uint64_t n, uint32_t launches, uint64_t bytes[2]);
 
 inline Status hip_status(hipError_t e, const char *what)
 {

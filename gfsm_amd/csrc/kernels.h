@@ -279,6 +279,13 @@ static_assert(512 + 32 * kNoAllocTiles <= kNoAllocWorkBytes, "scratch workspace 
 // current; its workspace part cleared on `s`), then cached.
 hipError_t noalloc_scratch(int device, hipStream_t s, char **ptr);
 
+// Traffic calibration (kernels of known footprint).  mode 0: dst[i] =
+// src[i] + 1 over n words (n words read and written); mode 1: one word from
+// each of n distinct 128-byte lines of src (n a power of two, src holding 32n
+// words) written to dst[0..n); mode 2: words 0 and 16 of each of those lines
+// (both 64-byte halves) written to dst[0..2n).
+hipError_t launch_calibrate(uint32_t mode, const uint32_t *src, uint32_t *dst, uint64_t n, hipStream_t s);
+
 // Zero the control block of a workspace (device kernel; safe under capture).
 hipError_t launch_ws_reset(ScanWS ws, Totals *tot, hipStream_t s);
 // Zero every tile word (needed once per 2^kEpochBits launches on a lane).

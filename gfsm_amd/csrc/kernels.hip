@@ -46,6 +46,7 @@ DEV gptr_t<T> gp(T *p)
 }
 
 constexpr uint32_t kWaves = kTile / 64;
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));  // one 16-byte store
 constexpr uint32_t kNodeLk = 4;  // node-setup lookups a thread keeps in flight
 
 // ---------------------------------------------------------------------------
@@ -798,12 +799,29 @@ DEV void expand_tile(const DevGraphView &g, const DP &dp, CP col, CP dcol,
                     nrow = ok_ >= io.cap ? 0 : (nrow < io.cap - ok_ ? nrow : io.cap - ok_);
                     const uint32_t words = (uint32_t)nrow * row_w;  // at most 64 * I
                     const auto dst = out_vec + ok_ * row_w;
-                    uint32_t v[I];
+                    // 16-byte stores for the body (a quarter of the store
+                    // instructions; the level is 256-byte aligned), single
+                    // words for the head before the first 16-byte boundary
+                    // and for the tail.
+                    const uint32_t head = (uint32_t)((4u - ((ok_ * row_w) & 3u)) & 3u);
+                    const uint32_t h = head < words ? head : words;
+                    const uint32_t nq = (words - h) >> 2;
+                    const uint32_t t0 = h + 4 * nq;
+                    const uint32_t hv = lane < h ? wst[lane] : 0u;
+                    const uint32_t tv = lane < words - t0 ? wst[t0 + lane] : 0u;
+                    u32x4 bv[(I + 3) / 4];
 #pragma unroll
-                    for (uint32_t i = 0; i < I; i++) v[i] = (lane + 64 * i < words) ? wst[lane + 64 * i] : 0u;
+                    for (uint32_t i = 0; i < (I + 3) / 4; i++) {
+                        const uint32_t c = lane + 64 * i;  // nq <= 16 * I groups
+                        const uint32_t *sw = wst + h + 4 * c;
+                        bv[i] = c < nq ? u32x4{sw[0], sw[1], sw[2], sw[3]} : u32x4{0u, 0u, 0u, 0u};
+                    }
+                    if (lane < h) dst[lane] = hv;
+                    if (lane < words - t0) dst[t0 + lane] = tv;
+                    const auto dst4 = (gptr_t<u32x4>)(dst + h);
 #pragma unroll
-                    for (uint32_t i = 0; i < I; i++)
-                        if (lane + 64 * i < words) dst[lane + 64 * i] = v[i];
+                    for (uint32_t i = 0; i < (I + 3) / 4; i++)
+                        if (lane + 64 * i < nq) dst4[lane + 64 * i] = bv[i];
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                     __builtin_amdgcn_wave_barrier();
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -1504,6 +1522,50 @@ hipError_t launch_seg_hash_build(const uint32_t *dir_ptr, const uint32_t *dir, u
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_seg_insert, dim3(4096), dim3(256), 0, s, dir_ptr, dir, V, slots, mask);
+    return hipGetLastError();
+}
+
+// Traffic calibration kernels of known HBM footprint (for scaling the byte
+// counters of a profiler run).  Stream: dst[i] = src[i] + 1 over n words.
+// Gather: dst[i] = src[32 * line(i)] for i < n, line(i) = i * odd mod n (n a
+// power of two): one word from each of n distinct 128-byte lines, in an order
+// that defeats the caches, written contiguously.
+__global__ void k_cal_stream(const uint32_t *__restrict__ src, uint32_t *__restrict__ dst, uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i] + 1u;
+}
+
+__global__ void k_cal_gather(const uint32_t *__restrict__ src, uint32_t *__restrict__ dst, uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t line = (i * 0x9E3779B97F4A7C15ull) & (n - 1);
+        dst[i] = src[line * 32];
+    }
+}
+
+// Both 64-byte halves of each line: dst[2i], dst[2i+1] = words 0 and 16 of
+// line(i).  Against k_cal_gather this tells whether a scattered read moves a
+// whole 128-byte line or only the 64-byte half it touches.
+__global__ void k_cal_gather2(const uint32_t *__restrict__ src, uint32_t *__restrict__ dst, uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t line = (i * 0x9E3779B97F4A7C15ull) & (n - 1);
+        dst[2 * i] = src[line * 32];
+        dst[2 * i + 1] = src[line * 32 + 16];
+    }
+}
+
+hipError_t launch_calibrate(uint32_t mode, const uint32_t *src, uint32_t *dst, uint64_t n, hipStream_t s)
+{
+    if (mode > 2 || (mode >= 1 && (n & (n - 1)))) return hipErrorInvalidValue;
+    const uint32_t grid = 8192;
+    if (mode == 0)
+        hipLaunchKernelGGL(k_cal_stream, dim3(grid), dim3(256), 0, s, src, dst, n);
+    else if (mode == 1)
+        hipLaunchKernelGGL(k_cal_gather, dim3(grid), dim3(256), 0, s, src, dst, n);
+    else
+        hipLaunchKernelGGL(k_cal_gather2, dim3(grid), dim3(256), 0, s, src, dst, n);
     return hipGetLastError();
 }
 

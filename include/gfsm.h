@@ -281,6 +281,14 @@ void        gfsm_server_stop(gfsm_server *srv);
  * totals[0] = number of non-zero values, totals[1] = sum.  Host arrays. */
 gfsm_status gfsm_selftest_scan(gfsm_ctx *ctx, const uint32_t *vals, uint64_t n, uint64_t *prefix,
                                uint32_t *idx, uint64_t totals[2]);
+/* (no counterpart) Kernels of known HBM footprint, for scaling a profiler's
+ * byte counters: mode 0 streams n words (dst[i] = src[i] + 1: 4n bytes read
+ * and written per launch); mode 1 gathers one word from each of n distinct
+ * 128-byte lines in a cache-defeating order (n a power of two: 128n bytes of
+ * lines read, 4n written per launch); mode 2 gathers words 0 and 16 (both
+ * 64-byte halves) of the same lines (128n read, 8n written).  Runs
+ * `launches` launches; bytes = {read, written} per launch. */
+gfsm_status gfsm_selftest_traffic(gfsm_ctx *ctx, uint32_t mode, uint64_t n, uint32_t launches, uint64_t bytes[2]);
 
 /* ---- tile shapes and the autotuner (no counterpart) ------------------------
  * The kernels' launch shapes, per context.  items[kind][d]: candidates per

@@ -268,3 +268,15 @@ def test_rowless_vertices_follow_the_rule(ctx):
         checked += 1
         with_rowless += bool(rowless & {b for _, b, _ in ent})
     assert checked >= 50 and with_rowless >= 20, (checked, with_rowless)
+
+
+def test_traffic_calibration_kernels(ctx):
+    """The calibration kernels report their footprint by construction (the
+    profiler's byte counters are scaled by them) and reject bad shapes."""
+    assert ctx.selftest_traffic(0, 1 << 20, 2) == (4 << 20, 4 << 20)
+    assert ctx.selftest_traffic(1, 1 << 16, 2) == (128 << 16, 4 << 16)
+    assert ctx.selftest_traffic(2, 1 << 16, 2) == (128 << 16, 8 << 16)
+    with pytest.raises(G.GfsmError):
+        ctx.selftest_traffic(1, 3 << 16, 1)  # the gather needs a power of two
+    with pytest.raises(G.GfsmError):
+        ctx.selftest_traffic(3, 1 << 10, 1)

@@ -741,8 +741,11 @@ struct Runner {
                 Status ks = kbegin();
                 if (!ks.good()) return ks;
             }
-            HIP_TRY(launch_expand(a, (uint32_t)std::min<uint64_t>(io.ntiles, (uint64_t)grid_cap * grid_mul),
-                                  lane.stream));
+            // Counting kinds deal their tiles by XCD when the grid is a
+            // multiple of the XCD count (kernels.hip, TileDeal).
+            uint64_t grid = std::min<uint64_t>(io.ntiles, (uint64_t)grid_cap * grid_mul);
+            if (kind != EXPAND_MID && kind != EXPAND_LAST_ROWS && grid >= 8) grid &= ~7ull;
+            HIP_TRY(launch_expand(a, (uint32_t)grid, lane.stream));
             res.stats.launches++;
             if (!first) res.stats.chunks++;
             first = false;

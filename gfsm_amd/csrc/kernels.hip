@@ -882,6 +882,32 @@ DEV void expand_tile(const DevGraphView &g, const DP &dp, CP col, CP dcol,
 }
 
 
+// Tiles of the kinds without a look-back, dealt so that runs of kXcdRun
+// consecutive tiles go to workgroups of one XCD: consecutive tiles expand
+// neighbouring nodes, often from the same generator segment, so the lines
+// one tile fetches into its XCD's L2 serve the next.  Workgroups are placed
+// on the 8 XCDs round-robin by workgroup index; run r of the launch belongs
+// to XCD r mod 8, and the workgroups of that XCD take its tiles in order.
+// Every tile is taken once for any grid; grids that are not a multiple of 8
+// deal plain round-robin.
+constexpr uint32_t kXcds = 8;
+constexpr uint64_t kXcdRun = 16;
+
+struct TileDeal {
+    uint64_t s, step, x;
+    bool xcd;
+    DEV TileDeal()
+    {
+        xcd = (gridDim.x % kXcds) == 0;
+        x = blockIdx.x % kXcds;
+        s = xcd ? blockIdx.x / kXcds : blockIdx.x;
+        step = xcd ? gridDim.x / kXcds : gridDim.x;
+    }
+    // tile of this workgroup's s-th share (increasing in s)
+    DEV uint64_t tile() const { return xcd ? ((s / kXcdRun) * kXcds + x) * kXcdRun + (s % kXcdRun) : s; }
+    DEV void next() { s += step; }
+};
+
 // Four waves per SIMD (at most 128 VGPRs): the kernel is bound by the latency
 // of dependent loads, and the extra resident workgroup hides more of it than
 // the few registers spilled at 8 candidates per thread cost (measured).
@@ -893,10 +919,11 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(4))) void
     const uint32_t ep = read_epoch(a.ws);
     uint64_t acc_cnt = 0, acc_h0 = 0, acc_h1 = 0;
     if (KIND == EXPAND_LAST_COUNT) {
-        // No tile waits on another, so tiles are dealt round-robin: a shared
+        // No tile waits on another, so tiles are dealt without tickets: a shared
         // ticket counter that every tile passes through caps this kind's
         // rate (measured: 1.7x faster without it).
-        for (uint64_t t = blockIdx.x; t < a.io.ntiles; t += gridDim.x) {
+        for (TileDeal dl; dl.tile() < a.io.ntiles; dl.next()) {
+            const uint64_t t = dl.tile();
             __syncthreads();  // the previous tile's readers of the tile LDS are done
             expand_tile<KIND, I>(a.g, a.dp, a.col, a.dcol, a.io, t, t, 0, a.ws, ep, a.pool_words, sh, smem, acc_cnt,
                                  acc_h0, acc_h1, false);
@@ -924,7 +951,8 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(8))) void
     __shared__ TileShared<I> sh;
     const uint32_t ep = read_epoch(a.ws);
     uint64_t acc_cnt = 0, acc_h0 = 0, acc_h1 = 0;
-    for (uint64_t t = blockIdx.x; t < a.io.ntiles; t += gridDim.x) {  // round-robin, as for LAST_COUNT
+    for (TileDeal dl; dl.tile() < a.io.ntiles; dl.next()) {  // dealt by XCD, as for LAST_COUNT
+        const uint64_t t = dl.tile();
         __syncthreads();
         expand_tile<EXPAND_LAST_BARE, I>(a.g, a.dp, a.col, a.dcol, a.io, t, t, 0, a.ws, ep, a.pool_words, sh, smem,
                                          acc_cnt, acc_h0, acc_h1, false);

@@ -342,3 +342,21 @@ def test_level_projection_tracks_the_oracle_on_random_graphs(mode, nvl):
 
 def oracle_levels(data, q, mode):
     return O.solve(to_oracle(data), to_oracle(q), mode=mode, rows=False).level
+
+
+def test_host_code_under_sanitizers(tmp_path):
+    """The parser, image loader, layout builder, generators and planner built
+    with AddressSanitizer and UBSan (tests/host_asan): valid inputs, malformed
+    text, damaged and crafted images and generated query plans run without a
+    sanitizer report and with the expected statuses."""
+    import shutil
+    import subprocess
+
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "host_asan")
+    subprocess.run(["make", "-s", "-C", here], check=True, capture_output=True)
+    r = subprocess.run([os.path.join(here, "_build", "asan_main"), str(tmp_path)], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr[-4000:]
+    assert "ERROR: AddressSanitizer" not in r.stderr and "runtime error" not in r.stderr

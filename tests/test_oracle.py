@@ -152,7 +152,15 @@ def test_kat2_triangle_requirements():
     assert [int(x) for x in c.packed()] == [2, 0x01000003, 0x01000006, 2, 0x01000000, 0x01000006, 2,
                                             0x01000000, 0x01000003]
     p = O.Plan(c).fields()
-    assert p["requirements"].tolist() == [0, 0, 0, 0, 1, 1, 0, 0, 1, 1, 1, 1]
+    # The reference's requirement words: {distance, relation} pairs where the
+    # relation is taken from the packed CCSR word, (rel << 24) & relMask
+    # (ccsr.cpp:148).  The oracle keeps the relation value itself (its CCSR
+    # splits the relation out of the word, so relations need not fit 8 bits);
+    # packed the reference's way, its table is KAT-2 word for word.
+    R = 0x01000000
+    req = p["requirements"].tolist()
+    packed = [v << 24 if i % 2 else v for i, v in enumerate(req)]
+    assert packed == [0, 0, 0, 0, 1, R, 0, 0, 1, R, 1, R]
     assert p["header"].tolist() == [0, 0, 0, 1, 4, 1, 2, 8, 1]
 
 

@@ -261,6 +261,7 @@ def serve_section(n):
             tk.append(srv.submit(q, root_lo=lo, root_hi=hi, max_cand=cap, max_depth_cand=last))
         ol = [srv.wait(x) for x in tk]
         times = [srv.times(x) for x in tk]
+        counters = srv.stats()
     finally:
         srv.stop()
     lat = [t2 - t0_ for t0_, t1, t2 in times]
@@ -294,6 +295,7 @@ def serve_section(n):
         "equal_saturated": f"{eq_sat}/{n}",
         "equal_open_loop": f"{eq_ol}/{n}",
         "embeddings": int(sum(m.count for m in sat)),
+        "admissions": {k: counters[k] for k in ("admitted", "waited_memory", "waited_slot")},
         "tuned_shape": {k: tuned[k] for k in ("batch_items", "batch_pool_words", "batch_wgs_per_cu")},
     }
 

@@ -98,15 +98,20 @@ class Stats(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("items", (C.c_uint32 * 32) * 3), ("pool_words", C.c_uint32), ("wgs_per_cu", C.c_uint32),
                 ("batch_items", C.c_uint32), ("batch_pool_words", C.c_uint32), ("batch_wgs_per_cu", C.c_uint32),
-                ("batch_slots", C.c_uint32), ("batch_steps", C.c_uint32), ("pad", C.c_uint32),
+                ("batch_slots", C.c_uint32), ("batch_steps", C.c_uint32), ("serve_steps", C.c_uint32),
                 ("batch_step_budget", C.c_uint64)]
 
     def as_dict(self):
         return {"items": [list(self.items[k]) for k in range(3)], "pool_words": self.pool_words,
                 "wgs_per_cu": self.wgs_per_cu, "batch_items": self.batch_items,
                 "batch_pool_words": self.batch_pool_words, "batch_wgs_per_cu": self.batch_wgs_per_cu,
-                "batch_slots": self.batch_slots, "batch_steps": self.batch_steps,
+                "batch_slots": self.batch_slots, "batch_steps": self.batch_steps, "serve_steps": self.serve_steps,
                 "batch_step_budget": self.batch_step_budget}
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("admitted", C.c_uint64), ("waited_memory", C.c_uint64),
+                ("waited_slot", C.c_uint64), ("completed", C.c_uint64)]
 
 
 class ServeOpts(C.Structure):
@@ -170,6 +175,7 @@ def lib():
         "gfsm_server_wait": [vp, u64, P(vp)],
         "gfsm_server_times": [vp, u64, P(C.c_double)],
         "gfsm_server_stop": [vp],
+        "gfsm_server_stats": [vp, P(ServerStats)],
         "gfsm_selftest_scan": [vp, pu32, u64, P(u64), pu32, P(u64)],
         "gfsm_selftest_traffic": [vp, u32, u64, u32, P(u64)],
         "gfsm_gen_er": [u32, C.c_double, u32, u32, u64, u32, P(vp)],
@@ -562,6 +568,13 @@ class Server:
         t = (C.c_double * 3)()
         _check(lib().gfsm_server_times(self._h, ticket, t), "times")
         return tuple(t)
+
+    def stats(self):
+        """Admission counters: submitted, admitted, waited_memory (held back
+        for projected memory while a slot was free), waited_slot, completed."""
+        st = ServerStats()
+        _check(lib().gfsm_server_stats(self._h, C.byref(st)), "server_stats")
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
 
     def stop(self):
         if getattr(self, "_h", None) and self._h.value:

@@ -165,12 +165,14 @@ BatchEngine::~BatchEngine()
         if (e) (void)hipEventDestroy(e);
     (void)hipFree(slots_d_);
     (void)hipFree(report_d_);
+    (void)hipFree(final_d_);
     (void)hipFree(tile_start_d_);
     (void)hipFree(ctl_d_);
     (void)hipFree(plans_d_);
     if (own_arena_) (void)hipFree(arena_);
     if (stage_h_) (void)hipHostFree(stage_h_);
     if (report_h_) (void)hipHostFree(report_h_);
+    if (final_h_) (void)hipHostFree(final_h_);
     if (step_cand_h_) (void)hipHostFree(step_cand_h_);
     if (n_admit_h_) (void)hipHostFree(n_admit_h_);
     if (admit_h_) (void)hipHostFree(admit_h_);
@@ -201,11 +203,13 @@ Status BatchEngine::init(const BatchShape &shape, uint64_t arena_bytes, char *ar
     if (!st.good()) return st;
     HIP_TRY(hipMalloc(&slots_d_, sizeof(BatchSlot) * n));
     HIP_TRY(hipMalloc(&report_d_, sizeof(SlotReport) * n));
+    HIP_TRY(hipMalloc(&final_d_, sizeof(SlotFinal) * n));
     HIP_TRY(hipMalloc(&tile_start_d_, sizeof(uint64_t) * (n + 1)));
     HIP_TRY(hipMalloc(&ctl_d_, sizeof(BatchCtl)));
     HIP_TRY(hipMalloc(&plans_d_, sizeof(QueryPlanDev) * n));
     HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&stage_h_), sizeof(BatchSlot) * n, hipHostMallocDefault));
     HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&report_h_), sizeof(SlotReport) * n, hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&final_h_), sizeof(SlotFinal) * n, hipHostMallocDefault));
     HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&step_cand_h_), sizeof(uint64_t), hipHostMallocDefault));
     HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&n_admit_h_), sizeof(uint32_t), hipHostMallocDefault));
     HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&admit_h_), sizeof(AdmitEntry) * n, hipHostMallocDefault));
@@ -217,6 +221,7 @@ Status BatchEngine::init(const BatchShape &shape, uint64_t arena_bytes, char *ar
     }
     HIP_TRY(hipMemcpy(slots_d_, stage_h_, sizeof(BatchSlot) * n, hipMemcpyHostToDevice));
     HIP_TRY(hipMemset(report_d_, 0, sizeof(SlotReport) * n));
+    HIP_TRY(hipMemset(final_d_, 0, sizeof(SlotFinal) * n));
     HIP_TRY(hipMemset(tile_start_d_, 0, sizeof(uint64_t) * (n + 1)));
     HIP_TRY(hipMemset(ctl_d_, 0, sizeof(BatchCtl)));
     if (arena) {
@@ -240,6 +245,7 @@ Status BatchEngine::init(const BatchShape &shape, uint64_t arena_bytes, char *ar
     cfg_.plans = plans_d_;
     cfg_.admit = admit_d_;
     cfg_.report = report_d_;
+    cfg_.final = final_d_;
     cfg_.tile_start = tile_start_d_;
     cfg_.ctl = ctl_d_;
     cfg_.nslots = n;
@@ -340,6 +346,7 @@ bool BatchEngine::try_admit(std::unique_ptr<BatchJob> &job, Status &st)
     if (off == UINT64_MAX) return false;
     job->region_off = off;
     job->region_bytes = want;
+    job->nroots = rhi - rlo;
     job->t_admit = now_ms();
 
     AdmitEntry &ae = admit_h_[n_admit_];
@@ -405,6 +412,10 @@ Status BatchEngine::replay()
     steps_ += K;
     HIP_TRY(hipMemcpyAsync(report_h_, report_d_, sizeof(SlotReport) * shape_.slots, hipMemcpyDeviceToHost,
                            lane_.stream));
+    // With the reports, the statistics of slots that finished (written once
+    // per finish): a harvest needs nothing more from the device.
+    HIP_TRY(hipMemcpyAsync(final_h_, final_d_, sizeof(SlotFinal) * shape_.slots, hipMemcpyDeviceToHost,
+                           lane_.stream));
     HIP_TRY(hipStreamSynchronize(lane_.stream));
     n_admit_ = 0;
     float ms = 0;
@@ -432,14 +443,11 @@ Status BatchEngine::harvest(const std::function<void(std::unique_ptr<BatchJob>, 
         if (st == SLOT_DONE || st == SLOT_CAP || st == SLOT_ERROR) fin.push_back(s);
     }
     if (fin.empty()) return Status::ok();
-    // One copy of the slot table (a few MB at most) rather than one per slot.
-    HIP_TRY(hipMemcpyAsync(stage_h_, slots_d_, sizeof(BatchSlot) * shape_.slots, hipMemcpyDeviceToHost,
-                           lane_.stream));
-    HIP_TRY(hipStreamSynchronize(lane_.stream));
     const double t = now_ms();
     for (uint32_t s : fin) {
         std::unique_ptr<BatchJob> job = std::move(jobs_[s]);
-        const BatchSlot &b = stage_h_[s];
+        const SlotReport &b = report_h_[s];
+        const SlotFinal &f = final_h_[s];
         Result r;
         const uint32_t q = job->plan.q;
         r.width = q;
@@ -448,11 +456,11 @@ Status BatchEngine::harvest(const std::function<void(std::unique_ptr<BatchJob>, 
         r.digest[1] = b.dg1;
         r.stats.q = q;
         r.stats.launches = b.steps;
-        r.stats.overflows = b.overflows;
-        r.stats.roots = b.nroots;
+        r.stats.overflows = f.overflows;
+        r.stats.roots = job->nroots;
         for (uint32_t k = 0; k < q && k < GFSM_MAXQ; k++) {
-            r.stats.level[k] = b.level[k];
-            r.stats.cand[k] = b.cand[k];
+            r.stats.level[k] = f.level[k];
+            r.stats.cand[k] = f.cand[k];
             if (k) r.stats.checks[k] = job->plan.dev.depth[k].nchk;
         }
         r.stats.alg_bytes = alg_bytes(job->plan, r.stats, job->opts.flags);

@@ -462,7 +462,7 @@ gfsm_status gfsm_tuning_get(gfsm_ctx *ctx, gfsm_tuning *t)
     t->batch_wgs_per_cu = u.batch_wgs_per_cu;
     t->batch_slots = u.batch_slots;
     t->batch_steps = u.batch_steps;
-    t->pad = 0;
+    t->serve_steps = u.serve_steps;
     t->batch_step_budget = u.batch_step_budget;
     return ok();
     GUARD_END
@@ -484,6 +484,7 @@ gfsm_status gfsm_tuning_set(gfsm_ctx *ctx, const gfsm_tuning *t)
         return fail(Status::err(GFSM_ERR_ARG, "workgroups per CU must be 1..32"));
     if (t->batch_slots < 1 || t->batch_slots > kMaxSlots)
         return fail(Status::err(GFSM_ERR_ARG, "batch slots must be 1..1024"));
+    if (t->serve_steps > 64) return fail(Status::err(GFSM_ERR_ARG, "serve_steps must be 0 (default 2) .. 64"));
     if (t->batch_steps < 1 || t->batch_steps > 64)
         return fail(Status::err(GFSM_ERR_ARG, "batch steps per replay must be 1..64"));
     if (t->batch_step_budget < (1ull << 16))
@@ -499,6 +500,7 @@ gfsm_status gfsm_tuning_set(gfsm_ctx *ctx, const gfsm_tuning *t)
     u.batch_wgs_per_cu = t->batch_wgs_per_cu;
     u.batch_slots = t->batch_slots;
     u.batch_steps = t->batch_steps;
+    u.serve_steps = t->serve_steps ? t->serve_steps : 2;
     u.batch_step_budget = t->batch_step_budget;
     return ok();
     GUARD_END
@@ -583,6 +585,13 @@ gfsm_status gfsm_server_times(gfsm_server *srv, uint64_t ticket, double t[3])
 {
     if (!srv || !t) return fail(Status::err(GFSM_ERR_ARG, "null argument"));
     Status st = server_times(*reinterpret_cast<Server *>(srv), ticket, t);
+    return st.good() ? ok() : fail(st);
+}
+
+gfsm_status gfsm_server_stats(gfsm_server *srv, gfsm_serve_stats *out)
+{
+    if (!srv || !out) return fail(Status::err(GFSM_ERR_ARG, "null argument"));
+    Status st = server_stats(*reinterpret_cast<Server *>(srv), *out);
     return st.good() ? ok() : fail(st);
 }
 

@@ -365,37 +365,6 @@ uint64_t alg_bytes(const Plan &plan, const gfsm_stats &st, uint32_t flags)
     return b;
 }
 
-Status match_single_vertex(const HostGraph &data, const Plan &plan, const gfsm_match_opts &o, Result &res)
-{
-    // A one-vertex query has no edges; the reference's root level lists every
-    // vertex that owns a CCSR row (HnGPU.cu:475-495), i.e. out-degree >= 1,
-    // and applies the vertex's self-loop requirements.
-    uint32_t lo = o.root_lo, hi = (o.root_hi || o.root_lo) ? o.root_hi : 0xFFFFFFFFu;
-    if (hi == 0) hi = 0xFFFFFFFFu;
-    res.count = 0;
-    for (uint32_t x = lo; x < data.V && x < hi; x++) {
-        if (data.degree(x) == 0) continue;
-        bool ok = true;
-        for (uint32_t r = 0; r < plan.dev.nroot_chk && ok; r++) {
-            uint32_t b, e;
-            data.segment(x, plan.dev.root_chk_rel[r], b, e);
-            ok = std::binary_search(data.nbr.begin() + b, data.nbr.begin() + e, x);
-        }
-        if (!ok) continue;
-        res.count++;
-        uint64_t h[2];
-        row_hash(&x, 1, h);
-        res.digest[0] += h[0];
-        res.digest[1] += h[1];
-        if (o.flags & (GFSM_ROWS | GFSM_SORTED)) res.rows.push_back(x);
-    }
-    res.stats.level[0] = res.count;
-    res.stats.roots = res.count;
-    if (o.max_rows && (o.flags & (GFSM_ROWS | GFSM_SORTED)) && res.count > o.max_rows)
-        return Status::err(GFSM_ERR_CAP, "more rows than max_rows");
-    return Status::ok();
-}
-
 namespace {
 
 // First chunk of a capped match = cap / kCapStartDiv candidates: each launch
@@ -858,6 +827,78 @@ Status selftest_scan(Ctx &ctx, const uint32_t *vals, uint64_t n, uint64_t *prefi
 
 namespace {
 
+// A one-vertex query on the device (k_single): the vertices of the root
+// window that own a row (out-degree >= 1) and pass the self-loop checks, in
+// id order.  With `fixed` (the allocation-free entry point) count mode only.
+Status run_single(Ctx &ctx, const HostGraph &data, const DevGraph &dg, const Plan &plan, const gfsm_match_opts &o,
+                  Result &res, Lane &L, const Arena *fixed)
+{
+    uint32_t lo = o.root_lo, hi = (o.root_hi || o.root_lo) ? o.root_hi : 0xFFFFFFFFu;
+    if (hi == 0) hi = 0xFFFFFFFFu;
+    hi = std::min(hi, data.V);
+    lo = std::min(lo, hi);
+    const bool rows = o.flags & (GFSM_ROWS | GFSM_SORTED);
+    SingleArgs a{};
+    a.g = dg.view;
+    a.lo = lo;
+    a.hi = hi;
+    a.nchk = plan.dev.nroot_chk;
+    std::memcpy(a.chk_rel, plan.dev.root_chk_rel, sizeof(a.chk_rel));
+    a.tot = L.tot_d;
+    a.ws = L.ws;
+    const uint64_t tc = (uint64_t)kTile * kRootItems;
+    a.ntiles = (hi - lo + tc - 1) / tc;
+    if (rows) {
+        if (fixed) return Status::err(GFSM_ERR_ARG, "rows need an allocation");
+        a.cap = hi - lo;
+        Status st = ensure_arena(ctx, std::max<uint64_t>(a.cap * 4, 256));
+        if (!st.good()) return st;
+        a.rows = reinterpret_cast<uint32_t *>(ctx.arena.base);
+        if (a.ntiles > L.ws.max_tiles) return Status::err(GFSM_ERR_ARG, "internal: root window too large");
+    }
+    if (fixed) {
+        HIP_TRY(launch_ws_clear(L.ws, L.stream));
+        HIP_TRY(launch_ws_reset(L.ws, L.tot_d, L.stream));
+    } else {
+        HIP_TRY(hipMemsetAsync(L.tot_d, 0, sizeof(Totals), L.stream));
+    }
+    HIP_TRY(hipEventRecord(L.ev[0], L.stream));
+    if (a.ntiles) {
+        HIP_TRY(L.before_launch());
+        const uint64_t grid = std::min<uint64_t>(a.ntiles, (uint64_t)std::max<uint32_t>(ctx.num_cus, 1) * ctx.tune.wgs_per_cu);
+        HIP_TRY(launch_single(a, (uint32_t)grid, L.stream));
+        res.stats.launches = 1;
+    }
+    HIP_TRY(hipEventRecord(L.ev[1], L.stream));
+    HIP_TRY(hipMemcpyAsync(L.tot_h, L.tot_d, sizeof(Totals), hipMemcpyDeviceToHost, L.stream));
+    HIP_TRY(hipStreamSynchronize(L.stream));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, L.ev[0], L.ev[1]));
+    res.stats.device_ms = ms;
+    if (rows) {
+        res.count = L.tot_h->keep;
+        res.rows.resize(res.count);
+        if (res.count)
+            HIP_TRY(hipMemcpy(res.rows.data(), a.rows, res.count * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        if (o.flags & GFSM_DIGEST)
+            for (uint32_t x : res.rows) {
+                uint64_t h[2];
+                row_hash(&x, 1, h);
+                res.digest[0] += h[0];
+                res.digest[1] += h[1];
+            }
+    } else {
+        res.count = L.tot_h->count;
+        res.digest[0] = L.tot_h->dg0;
+        res.digest[1] = L.tot_h->dg1;
+    }
+    res.stats.level[0] = res.count;
+    res.stats.roots = res.count;
+    res.stats.cand[0] = hi - lo;
+    if (o.max_rows && rows && res.count > o.max_rows) return Status::err(GFSM_ERR_CAP, "more rows than max_rows");
+    return Status::ok();
+}
+
 // One single-query match on `L` with the data graph resident: the depth-first
 // chunk walk from the roots, then the result.  With `fixed`, the levels live
 // in that arena and nothing is allocated (count mode only).
@@ -980,16 +1021,16 @@ Status match_plain(Ctx &ctx, const HostGraph &data, const HostGraph &query, cons
     if (!st.good()) return st;
     res.width = plan.q;
     res.stats.q = plan.q;
-    if (plan.q == 1) {
-        st = match_single_vertex(data, plan, o, res);
-        res.stats.host_ms = now_ms() - t0;
-        return st;
-    }
     std::lock_guard<std::mutex> lk(ctx.mu);
     HIP_TRY(hipSetDevice(ctx.device));
     DevGraph *dg = nullptr;
     st = upload_graph(ctx, data, dg);
     if (!st.good()) return st;
+    if (plan.q == 1) {
+        st = run_single(ctx, data, *dg, plan, o, res, ctx.lane, nullptr);  // rows returned, never streamed
+        res.stats.host_ms = now_ms() - t0;
+        return st;
+    }
     st = run_match(ctx, data, *dg, plan, o, res, stream, ctx.lane, nullptr);
     res.stats.host_ms = now_ms() - t0;
     return st;
@@ -1011,11 +1052,6 @@ Status match_noalloc(Ctx &ctx, const HostGraph &data, const HostGraph &query, co
     if (!st.good()) return st;
     res.width = plan.q;
     res.stats.q = plan.q;
-    if (plan.q == 1) {
-        st = match_single_vertex(data, plan, o, res);
-        res.stats.host_ms = now_ms() - t0;
-        return st;
-    }
     DevGraph *dg = nullptr;
     {
         std::lock_guard<std::mutex> lk(g_dev_mu);
@@ -1045,7 +1081,8 @@ Status match_noalloc(Ctx &ctx, const HostGraph &data, const HostGraph &query, co
     Arena levels;
     levels.base = p + kNoAllocWorkBytes;
     levels.bytes = kNoAllocLevelBytes;
-    st = run_match(ctx, data, *dg, plan, o, res, nullptr, L, &levels);  // L borrows: nothing to release
+    st = plan.q == 1 ? run_single(ctx, data, *dg, plan, o, res, L, &levels)
+                     : run_match(ctx, data, *dg, plan, o, res, nullptr, L, &levels);  // L borrows: nothing to release
     res.stats.host_ms = now_ms() - t0;
     return st;
 }

@@ -75,7 +75,8 @@ struct Tuning {
     uint32_t batch_pool_words = 14336;
     uint32_t batch_wgs_per_cu = 4;
     uint32_t batch_slots = 256;
-    uint32_t batch_steps = 4;  // steps per captured replay (the server uses at most 2)
+    uint32_t batch_steps = 4;  // steps per captured replay of gfsm_match_batch
+    uint32_t serve_steps = 2;  // ... of the server (short: a finished slot is refilled at the end of its replay)
     uint64_t batch_step_budget = 1ull << 26;
     Tuning();
 };
@@ -142,7 +143,6 @@ std::mutex &noalloc_mutex(int device);
 Status plan_for(const HostGraph &data, const HostGraph &query, const gfsm_match_opts &o, Plan &plan);
 
 // Matcher helpers shared with the batched and serving paths.
-Status match_single_vertex(const HostGraph &data, const Plan &plan, const gfsm_match_opts &opts, Result &res);
 void root_range(const HostGraph &data, const Plan &plan, uint32_t root_lo, uint32_t root_hi, uint64_t &lo,
                 uint64_t &hi);
 Status selftest_scan(Ctx &ctx, const uint32_t *vals, uint64_t n, uint64_t *prefix, uint32_t *idx,

@@ -125,6 +125,23 @@ struct RootArgs {
     ScanWS ws;
 };
 
+// One-vertex query (the reference's root level with no edges to expand,
+// HnGPU.cu:475-495): every vertex id in [lo, hi) that has an out-entry and
+// carries each self-loop relation.  Rows mode writes the kept ids in id
+// order (chained scan) to rows[0, cap); count mode sums count and digest.
+struct SingleArgs {
+    DevGraphView g;
+    uint32_t lo, hi;
+    uint32_t nchk;
+    uint32_t chk_rel[GFSM_MAXCHK];
+    uint32_t *rows;            // rows mode: [cap] kept ids, else null
+    uint64_t cap;
+    Totals *tot;               // keep (rows mode) or count and digest
+    uint64_t ntiles;
+    ScanWS ws;
+};
+hipError_t launch_single(const SingleArgs &a, uint32_t grid, hipStream_t s);
+
 constexpr uint32_t kTile = 256;     // threads per workgroup
 constexpr uint32_t kItems = 8;      // default candidates per thread per tile
 constexpr uint32_t kRootItems = 4;  // root level (single-query path)
@@ -176,6 +193,14 @@ struct SlotReport {
     uint32_t state;
     uint32_t steps;
     uint64_t count, dg0, dg1;
+};
+
+// Statistics of a finished slot, written by the scheduler when the slot
+// finishes and copied with the reports, so a harvest needs no copy of the
+// slot table.
+struct SlotFinal {
+    uint32_t overflows, pad;
+    uint64_t level[GFSM_MAXQ], cand[GFSM_MAXQ];
 };
 
 struct BatchSlot {
@@ -234,6 +259,7 @@ struct BatchCfg {
     QueryPlanDev *plans;          // [nslots], slot s's plan (BatchSlot::plan points here)
     const AdmitEntry *admit;      // [nslots] admission queue
     SlotReport *report;           // [nslots]
+    SlotFinal *final;             // [nslots], written when a slot finishes
     uint64_t *tile_start;         // [nslots + 1]
     BatchCtl *ctl;
     uint32_t nslots;

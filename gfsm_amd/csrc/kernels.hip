@@ -474,6 +474,56 @@ __global__ __launch_bounds__(kTile) void k_root(RootArgs a)
     finish_launch(a.ws);
 }
 
+// One-vertex queries: a vertex qualifies when its directory is non-empty
+// (it owns a segment, so it has an out-entry) and it lies in its own segment
+// of every self-loop relation.  The row hash of a one-column row {x} is
+// mix(mix(seed ^ x)) per seed.
+template <bool ROWS>
+__global__ __launch_bounds__(kTile) void k_single(SingleArgs a)
+{
+    constexpr uint32_t I = kRootItems;
+    __shared__ TileShared<I> sh;
+    const uint32_t ep = read_epoch(a.ws);
+    uint64_t acc = 0, h0 = 0, h1 = 0;
+    if (ROWS) tickets_init(a.ws, &sh.next);
+    for (uint64_t t = ROWS ? 0 : blockIdx.x;; t += ROWS ? 0 : gridDim.x) {
+        if (ROWS) t = tickets_take(&sh.next);
+        if (t >= a.ntiles) break;
+        bool keep[I];
+        uint32_t x[I], zero[I];
+#pragma unroll
+        for (uint32_t k = 0; k < I; k++) {
+            const uint64_t v = (uint64_t)a.lo + t * (kTile * I) + k * kTile + threadIdx.x;
+            x[k] = (uint32_t)v;
+            zero[k] = 0;
+            keep[k] = v < a.hi && a.g.dir_ptr[v + 1] > a.g.dir_ptr[v];
+        }
+        for (uint32_t r = 0; r < a.nchk; r++) {
+            uint2 sg[I];
+            seg_lookup_n<I>(a.g, x, a.chk_rel[r], keep, sg);
+            seg_contains_n<I>(a.g.nbr, sg, x, keep);
+        }
+        if (ROWS) {
+            uint64_t kk[I], off[I], rk, rc;
+            tile_scan<I>(keep, zero, t, 0, a.ws, ep, sh.sm, &sh.next, kk, off, rk, rc);
+#pragma unroll
+            for (uint32_t k = 0; k < I; k++)
+                if (keep[k] && kk[k] < a.cap) gp(a.rows)[kk[k]] = x[k];
+            if (t == a.ntiles - 1 && threadIdx.x == 0) gp(a.tot)->keep = rk;
+        } else {
+#pragma unroll
+            for (uint32_t k = 0; k < I; k++) {
+                if (!keep[k]) continue;
+                acc++;
+                h0 += mix64(mix64(0x9E3779B97F4A7C15ull ^ (uint64_t)x[k]));
+                h1 += mix64(mix64(0xD1B54A32D192ED03ull ^ (uint64_t)x[k]));
+            }
+        }
+    }
+    if (!ROWS) flush_sums(acc, h0, h1, a.tot, sh.red);
+    finish_launch(a.ws);
+}
+
 // ---------------------------------------------------------------------------
 // one depth: expand + filter + compact + next level
 // ---------------------------------------------------------------------------
@@ -1188,7 +1238,16 @@ __global__ __launch_bounds__(kTile) void k_batch_schedule(BatchCfg cfg)
         const uint32_t s = s0 + u;
         if (s < cfg.nslots) {
             BatchSlot &B = cfg.slots[s];
+            const uint32_t before = B.state;
             nt[u] = schedule_slot(cfg, B);
+            if (B.state >= SLOT_DONE && before < SLOT_DONE) {  // just finished: its statistics
+                SlotFinal &f = cfg.final[s];
+                f.overflows = B.overflows;
+                for (uint32_t k = 0; k < GFSM_MAXQ; k++) {
+                    f.level[k] = B.level[k];
+                    f.cand[k] = B.cand[k];
+                }
+            }
             my_tiles += nt[u];
             my_run += (B.state == SLOT_RUN) ? 1u : 0u;
             SlotReport rp;
@@ -1494,6 +1553,15 @@ hipError_t launch_tile_index(const uint64_t *off, uint64_t S, uint64_t g_begin, 
     const uint64_t keys_per_wg = kTile / 32;
     const uint64_t grid = (ntiles + keys_per_wg) / keys_per_wg;  // ntiles + 1 keys
     hipLaunchKernelGGL(k_tile_index, dim3((uint32_t)grid), dim3(kTile), 0, s, off, S, g_begin, g_end, tc, ntiles, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_single(const SingleArgs &a, uint32_t grid, hipStream_t s)
+{
+    if (a.rows)
+        hipLaunchKernelGGL(k_single<true>, dim3(grid), dim3(kTile), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_single<false>, dim3(grid), dim3(kTile), 0, s, a);
     return hipGetLastError();
 }
 

@@ -37,6 +37,7 @@ struct Server {
     static constexpr size_t kTakenKept = 1u << 20;
     std::map<uint64_t, Item> items;
     std::deque<uint64_t> taken_untimed;  // taken tickets whose timestamps were not read yet
+    gfsm_serve_stats counters{};        // guarded by mu
     std::deque<std::unique_ptr<BatchJob>> incoming;  // submitted, not yet seen by the worker
     uint64_t next = 1;
     bool stop = false;
@@ -55,6 +56,7 @@ struct Server {
             it.res.reset(new Result(std::move(r)));
             it.t[2] = now();
             it.done = true;
+            counters.completed++;
         }
         cv_done.notify_all();
     }
@@ -100,7 +102,12 @@ static void worker_main(Server *s)
             }
             Status st;
             const double t = s->now();
+            const uint32_t free_slots = s->eng->free_slots();
             if (!s->eng->try_admit(pending.front(), st)) {
+                if (st.good()) {  // waits: for projected memory while a slot is free, or for a slot
+                    if (free_slots) pending.front()->waited_memory = true;
+                    else pending.front()->waited_slot = true;
+                }
                 if (!st.good()) {  // can never fit
                     Result r;
                     r.status = st.code;
@@ -110,9 +117,12 @@ static void worker_main(Server *s)
                 }
                 break;
             }
-            pending.pop_front();
             std::lock_guard<std::mutex> lk(s->mu);
             s->items[id].t[1] = t;
+            s->counters.admitted++;
+            s->counters.waited_memory += job.waited_memory;
+            s->counters.waited_slot += job.waited_slot;
+            pending.pop_front();
         }
         if (s->eng->running()) {
             Status st = s->eng->advance(done);
@@ -176,7 +186,7 @@ Status server_start(Ctx &ctx, const HostGraph &data, const gfsm_serve_opts &opts
     sh.pool_words = ctx.tune.batch_pool_words;
     sh.wgs_per_cu = ctx.tune.batch_wgs_per_cu;
     // Short replays: a finished slot is refilled at the end of its replay.
-    sh.steps_per_replay = std::min<uint32_t>(ctx.tune.batch_steps, 2);
+    sh.steps_per_replay = std::max<uint32_t>(ctx.tune.serve_steps, 1);
     sh.step_budget = ctx.tune.batch_step_budget;
     if (sh.slots > kMaxSlots) return Status::err(GFSM_ERR_ARG, "max_inflight must be at most 1024");
     const uint64_t budget = opts.mem_budget ? opts.mem_budget : ctx.budget;
@@ -225,6 +235,7 @@ Status server_submit(Server &s, const HostGraph &query, const gfsm_match_opts *o
             pin_graph(query);  // released when its result is recorded
             s.incoming.push_back(std::move(job));
         }
+        s.counters.submitted++;
     }
     s.cv_work.notify_all();
     return Status::ok();
@@ -254,6 +265,13 @@ Status server_take(Server &s, uint64_t ticket, bool wait, Result &out)
             s.taken_untimed.pop_front();
         }
     }
+    return Status::ok();
+}
+
+Status server_stats(Server &s, gfsm_serve_stats &out)
+{
+    std::lock_guard<std::mutex> lk(s.mu);
+    out = s.counters;
     return Status::ok();
 }
 

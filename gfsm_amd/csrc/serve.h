@@ -40,6 +40,9 @@ struct BatchJob {
     gfsm_match_opts opts{};
     Plan plan;
     uint64_t region_off = 0, region_bytes = 0;
+    uint64_t nroots = 0;                 // root candidates of its window
+    bool waited_memory = false;          // the server held it back for projected memory (a slot was free)
+    bool waited_slot = false;            // ... for a free slot
     double model_bytes = 0;  // projected level bytes before calibration
     double t_admit = 0;
 };
@@ -103,6 +106,7 @@ private:
     BatchSlot *slots_d_ = nullptr;
     BatchSlot *stage_h_ = nullptr;       // pinned staging for admissions
     SlotReport *report_d_ = nullptr, *report_h_ = nullptr;
+    SlotFinal *final_d_ = nullptr, *final_h_ = nullptr;
     uint64_t *tile_start_d_ = nullptr;
     BatchCtl *ctl_d_ = nullptr;
     QueryPlanDev *plans_d_ = nullptr;
@@ -148,6 +152,7 @@ Status server_start(Ctx &ctx, const HostGraph &data, const gfsm_serve_opts &opts
 Status server_submit(Server &s, const HostGraph &query, const gfsm_match_opts *opts, uint64_t &ticket);
 Status server_take(Server &s, uint64_t ticket, bool wait, Result &out);
 Status server_times(Server &s, uint64_t ticket, double t[3]);
+Status server_stats(Server &s, gfsm_serve_stats &out);
 void server_stop(Server *s);
 
 }  // namespace gfsm

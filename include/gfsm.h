@@ -273,6 +273,18 @@ gfsm_status gfsm_server_wait(gfsm_server *srv, uint64_t ticket, gfsm_result **r)
  * timestamps read (then GFSM_ERR_ARG); of taken results whose timestamps are
  * never read, the server keeps the most recent 2^20 records. */
 gfsm_status gfsm_server_times(gfsm_server *srv, uint64_t ticket, double t[3]);
+/* Admission counters since the server started.  A query admitted to the
+ * batch engine counts in waited_memory when the server held it back at
+ * least once because its projected memory did not fit while a slot was free,
+ * and in waited_slot when it was held back for a free slot. */
+typedef struct gfsm_serve_stats {
+    uint64_t submitted;      /* tickets handed out                                   */
+    uint64_t admitted;       /* queries run on the batch engine                       */
+    uint64_t waited_memory;  /* of those, held back for projected memory              */
+    uint64_t waited_slot;    /* of those, held back for a free slot                   */
+    uint64_t completed;      /* results recorded (any status)                         */
+} gfsm_serve_stats;
+gfsm_status gfsm_server_stats(gfsm_server *srv, gfsm_serve_stats *out);
 void        gfsm_server_stop(gfsm_server *srv);
 
 /* ---- self test ------------------------------------------------------------
@@ -309,7 +321,7 @@ typedef struct gfsm_tuning {
     uint32_t batch_wgs_per_cu;
     uint32_t batch_slots;
     uint32_t batch_steps;        /* steps per captured replay (1..64)                           */
-    uint32_t pad;
+    uint32_t serve_steps;        /* ... of a server started afterwards (0 = default 2; <= 64)   */
     uint64_t batch_step_budget;  /* candidates of one step over all running queries            */
 } gfsm_tuning;
 gfsm_status gfsm_tuning_get(gfsm_ctx *ctx, gfsm_tuning *t);

@@ -236,3 +236,26 @@ def test_server_ticket_lifetime_and_graph_pins(er_vlabel):
         srv.submit(G.Graph.walk_query(er_vlabel, 3, seed=1))
     assert ei.value.code == 2
     srv.stop()
+
+
+@pytest.mark.gpu
+def test_server_admission_waits_on_projected_memory(ctx):
+    """A tight memory budget with many free slots: the server holds queries
+    back because their projected levels do not fit (not for want of a slot),
+    admits them as earlier queries release memory, and every result equals
+    the same query run alone."""
+    data = G.Graph.er(3000, 8.0, nvlabels=2, nelabels=1, seed=7700, mode=G.MODE_VLABEL)
+    qs = _queries(data, [5, 6, 5, 6, 7, 5, 6, 7] * 5, seed=7701)
+    alone = [ctx.match(data, q, digest=True, allow_partial=True, max_depth_cand=1 << 22) for q in qs]
+    budget = 256 << 20  # regions of up to budget / 16 each: at most ~16 queries fit at once
+    srv = G.Server(ctx, data, mem_budget=budget, max_inflight=128, digest=True)
+    try:
+        tk = [srv.submit(q, max_depth_cand=1 << 22) for q in qs]
+        got = [srv.wait(t) for t in tk]
+        st = srv.stats()
+    finally:
+        srv.stop()
+    assert [(m.count, m.digest, m.status) for m in got] == [(a.count, a.digest, a.status) for a in alone]
+    assert st["submitted"] == st["completed"] == len(qs)
+    assert st["admitted"] == len(qs)
+    assert st["waited_memory"] > 0 and st["waited_slot"] == 0

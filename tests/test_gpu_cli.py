@@ -9,7 +9,8 @@ import numpy as np
 import pytest
 
 import gfsm_amd as G
-from helpers import oracle_solve
+import oracle as O
+from helpers import assert_same, oracle_solve, to_oracle
 
 pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -88,10 +89,15 @@ def test_binary_image_matches_like_the_text_graph(ctx, tmp_path):
     text = G.Graph.load(dp, mode=G.MODE_VLABEL)
     text.save_bin(img)
     image = G.Graph.load_bin(img)
+    od = O.Edges.load(dp)  # the oracle parses the same text file
     for s in (3, 4, 5, 6):
         q = G.Graph.walk_query(text, s, seed=880 + s)
         a, b = ctx.match(text, q, digest=True), ctx.match(image, q, digest=True)
         assert (a.count, a.digest) == (b.count, b.digest)
+        # the image-loaded graph against the oracle: count, digest and sorted rows
+        ref = O.solve(od, to_oracle(q), mode=G.MODE_VLABEL, rows=True)
+        assert_same(ctx.match(image, q, rows=True, sorted=True, digest=True), ref)
+        assert (b.count, b.digest) == (ref.count, ref.digest)
     q = G.Graph.walk_query(text, 4, seed=900)
     q.write(qp)
     cache = tmp_path / "cache.bin"

@@ -280,3 +280,48 @@ def test_traffic_calibration_kernels(ctx):
         ctx.selftest_traffic(1, 3 << 16, 1)  # the gather needs a power of two
     with pytest.raises(G.GfsmError):
         ctx.selftest_traffic(3, 1 << 10, 1)
+
+
+def test_one_vertex_queries_on_the_device(ctx):
+    """One-vertex queries run on the device kernel: the vertices with an
+    out-entry (the reference's root level lists rows only) carrying every
+    self-loop relation, in id order; count, digest and rows against the
+    oracle, over the whole graph and over a root window, in both modes, and
+    through the allocation-free entry point."""
+    n = 300
+    rng = np.random.default_rng(33)
+    src, dst, lab = [], [], []
+    for a in range(n):
+        if rng.random() < 0.3:  # self loops with labels 1..2
+            src.append(a); dst.append(a); lab.append(int(rng.integers(1, 3)))
+    for a in range(n):
+        for b in range(a + 1, n):
+            if rng.random() < 0.02:
+                l = int(rng.integers(1, 3))
+                src += [a, b]; dst += [b, a]; lab += [l, l]
+    vl = [int(x) for x in rng.integers(1, 4, n)]
+    for mode in (G.MODE_REF, G.MODE_VLABEL):
+        data = G.Graph.from_edges(n, src, dst, lab, vlabel=vl, mode=mode)
+        ctx.upload(data)
+        od = O.Edges.from_arrays(n, src, dst, lab, vl)
+        queries = [([], [], []), ([0], [0], [1]), ([0], [0], [2])]  # plain vertex, self loop of label 1 or 2
+        for qs, qd, ql in queries:
+            q = G.Graph.from_edges(1, qs, qd, ql, vlabel=[int(vl[7])], mode=mode)
+            oq = O.Edges.from_arrays(1, qs, qd, ql, [int(vl[7])])
+            for lo, hi in ((0, 0), (40, 170)):
+                if qs:
+                    ref = O.solve(od, oq, mode=mode, rows=True, root_lo=lo, root_hi=hi if hi else 0xFFFFFFFF)
+                else:
+                    # no edge at all: the oracle's compiler rejects a query without a row
+                    # (as the reference's does); the rule lists every vertex with an out-entry
+                    ids = sorted({a for a in src if lo <= a < (hi or n)})
+                    ref = type("R", (), {})()
+                    ref.count, ref.width = len(ids), 1
+                    ref.rows = np.array(ids, np.uint32).reshape(-1, 1)
+                    hs = [O.row_hash([x]) for x in ids]
+                    ref.digest = (sum(h[0] for h in hs) % (1 << 64), sum(h[1] for h in hs) % (1 << 64))
+                m = ctx.match(data, q, rows=True, sorted=True, digest=True, root_lo=lo, root_hi=hi)
+                assert_same(m, ref)
+                c = ctx.match(data, q, digest=True, root_lo=lo, root_hi=hi)
+                assert (c.count, c.digest) == (ref.count, ref.digest)
+                assert ctx.match_noalloc(data, q, digest=True, root_lo=lo, root_hi=hi) == (0, ref.count, ref.digest)

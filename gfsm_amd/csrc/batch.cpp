@@ -125,8 +125,13 @@ double BatchEngine::model_bytes(const Plan &plan, uint64_t nroots, uint64_t g1) 
 // from finished queries, with headroom, clamped to [min, max] region.
 uint64_t BatchEngine::project_bytes(const Plan &plan, uint64_t nroots, uint64_t g1) const
 {
-    const double b = model_bytes(plan, nroots, g1) * calib_ * 1.25;
-    const uint64_t lo = min_region(plan.q);
+    return project_model(plan.q, model_bytes(plan, nroots, g1));
+}
+
+uint64_t BatchEngine::project_model(uint32_t q, double model) const
+{
+    const double b = model * calib_ * 1.25;
+    const uint64_t lo = min_region(q);
     const uint64_t hi = std::max(lo, max_region_);
     if (!(b < (double)hi)) return hi;
     return std::max(lo, round_up((uint64_t)b, kRegionAlign));
@@ -315,6 +320,19 @@ void BatchEngine::carve(BatchSlot &s, const Plan &plan, char *base, uint64_t byt
     }
 }
 
+void BatchEngine::model_job(const HostGraph &data, BatchJob &job)
+{
+    root_range(data, job.plan, job.opts.root_lo, job.opts.root_hi, job.rlo, job.rhi);
+    job.g1 = 0;
+    for (uint64_t i = job.rlo; i < job.rhi; i++) job.g1 += data.rel_cnt[i];
+    double nodes[GFSM_MAXQ], cand[GFSM_MAXQ];
+    project_levels(data, job.plan, job.rhi - job.rlo, job.g1, nodes, cand);
+    double bytes = 0;
+    for (uint32_t k = 0; k + 1 < job.plan.q; k++) bytes += nodes[k] * node_bytes(k);
+    job.model_bytes = bytes;
+    job.modeled = true;
+}
+
 bool BatchEngine::try_admit(std::unique_ptr<BatchJob> &job, Status &st)
 {
     st = Status::ok();
@@ -326,10 +344,8 @@ bool BatchEngine::try_admit(std::unique_ptr<BatchJob> &job, Status &st)
         }
     if (slot == UINT32_MAX) return false;
     const Plan &plan = job->plan;
-    uint64_t rlo, rhi;
-    root_range(data_, plan, job->opts.root_lo, job->opts.root_hi, rlo, rhi);
-    uint64_t g1 = 0;
-    for (uint64_t i = rlo; i < rhi; i++) g1 += data_.rel_cnt[i];
+    if (!job->modeled) model_job(data_, *job);
+    const uint64_t rlo = job->rlo, rhi = job->rhi;
     const uint64_t lo = min_region(plan.q);
     if (lo > arena_bytes_) {
         st = Status::err(GFSM_ERR_NOMEM, "batch arena too small for this query");
@@ -339,8 +355,7 @@ bool BatchEngine::try_admit(std::unique_ptr<BatchJob> &job, Status &st)
     // chunks), more when the projection asks for it, at most max_region.
     const uint64_t share = arena_bytes_ / shape_.slots / kRegionAlign * kRegionAlign;
     const uint64_t cap = std::min(max_region_, arena_bytes_);
-    job->model_bytes = model_bytes(plan, rhi - rlo, g1);
-    uint64_t want = std::min(cap, std::max(share, project_bytes(plan, rhi - rlo, g1)));
+    uint64_t want = std::min(cap, std::max(share, project_model(plan.q, job->model_bytes)));
     want = std::max(lo, want);
     const uint64_t off = regions_.take(want);
     if (off == UINT64_MAX) return false;

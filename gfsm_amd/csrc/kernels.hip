@@ -1113,8 +1113,19 @@ DEV uint32_t schedule_slot(const BatchCfg &cfg, BatchSlot &B)
     }
 
     // Next step: the deepest depth with candidates left, else more roots.
-    int32_t d = (int32_t)q - 1;
-    while (d >= 1 && B.lo[d] >= B.hi[d]) d--;
+    // Every depth's range is read at once (independent loads in flight
+    // together) rather than one depth at a time: the scheduler is one
+    // workgroup, and its dependent loads are on the critical path of every
+    // step.
+    int32_t d = 0;
+    {
+        bool left[GFSM_MAXQ];
+#pragma unroll
+        for (uint32_t k = 1; k < GFSM_MAXQ; k++) left[k] = k < q && B.lo[k] < B.hi[k];
+#pragma unroll
+        for (uint32_t k = 1; k < GFSM_MAXQ; k++)
+            if (left[k]) d = (int32_t)k;
+    }
     uint32_t ntiles = 0;
     B.tot = Totals{};
     // Deterministic cap: depth k tests only its first max_depth_cand
@@ -1199,12 +1210,6 @@ DEV uint32_t schedule_slot(const BatchCfg &cfg, BatchSlot &B)
     return ntiles;
 }
 
-// Copy `words` 32-bit words with all threads of the workgroup.
-DEV void block_copy(uint32_t *__restrict__ dst, const uint32_t *__restrict__ src, uint32_t words)
-{
-    for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) dst[i] = src[i];
-}
-
 // One workgroup: install queued admissions, schedule every slot, then lay
 // the slots' tiles end to end.
 __global__ __launch_bounds__(kTile) void k_batch_schedule(BatchCfg cfg)
@@ -1214,14 +1219,19 @@ __global__ __launch_bounds__(kTile) void k_batch_schedule(BatchCfg cfg)
     const uint32_t n_admit = cfg.ctl->n_admit;
     if (n_admit) {
         static_assert(sizeof(BatchSlot) % 4 == 0 && sizeof(QueryPlanDev) % 4 == 0, "word copies");
-        for (uint32_t e = 0; e < n_admit && e < cfg.nslots; e++) {
+        // All entries' words at once, striped over the workgroup, so the
+        // copies' loads overlap instead of one entry after another.
+        constexpr uint32_t SW = sizeof(BatchSlot) / 4, PW = sizeof(QueryPlanDev) / 4;
+        const uint32_t ne = n_admit < cfg.nslots ? n_admit : cfg.nslots;
+        for (uint32_t i = tid; i < ne * (SW + PW); i += blockDim.x) {
+            const uint32_t e = i / (SW + PW), w = i - e * (SW + PW);
             const AdmitEntry &a = cfg.admit[e];
             const uint32_t s = a.slot;
             if (s >= cfg.nslots) continue;
-            block_copy(reinterpret_cast<uint32_t *>(cfg.slots + s), reinterpret_cast<const uint32_t *>(&a.s),
-                       sizeof(BatchSlot) / 4);
-            block_copy(reinterpret_cast<uint32_t *>(cfg.plans + s), reinterpret_cast<const uint32_t *>(&a.plan),
-                       sizeof(QueryPlanDev) / 4);
+            if (w < SW)
+                reinterpret_cast<uint32_t *>(cfg.slots + s)[w] = reinterpret_cast<const uint32_t *>(&a.s)[w];
+            else
+                reinterpret_cast<uint32_t *>(cfg.plans + s)[w - SW] = reinterpret_cast<const uint32_t *>(&a.plan)[w - SW];
         }
         // The copies are read back below by other threads of this workgroup.
         __threadfence();

@@ -13,6 +13,8 @@
 #include "serve.h"
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <chrono>
 
 namespace gfsm {
@@ -64,6 +66,10 @@ struct Server {
 
 static void worker_main(Server *s)
 {
+    // GFSM_BATCH_TRACE: seconds spent admitting and advancing, at the end.
+    const bool trace = std::getenv("GFSM_BATCH_TRACE") != nullptr;
+    double t_admit = 0, t_advance = 0;
+    uint64_t n_advance = 0;
     std::deque<std::unique_ptr<BatchJob>> pending;
     // A job's query stays pinned until its result is recorded.
     auto done = [s](std::unique_ptr<BatchJob> job, Result &r) {
@@ -81,9 +87,15 @@ static void worker_main(Server *s)
                 pending.push_back(std::move(s->incoming.front()));
                 s->incoming.pop_front();
             }
-            if (s->stop && pending.empty() && s->eng->running() == 0) return;
+            if (s->stop && pending.empty() && s->eng->running() == 0) {
+                if (trace)
+                    std::fprintf(stderr, "server worker: admission %.3f s, %llu advances %.3f s\n", t_admit,
+                                 (unsigned long long)n_advance, t_advance);
+                return;
+            }
         }
         // Admission in arrival order while a slot and projected memory are free.
+        const double ta = trace ? s->now() : 0;
         while (!pending.empty()) {
             BatchJob &job = *pending.front();
             const uint64_t id = job.id;
@@ -124,8 +136,14 @@ static void worker_main(Server *s)
             s->counters.waited_slot += job.waited_slot;
             pending.pop_front();
         }
+        if (trace) t_admit += s->now() - ta;
         if (s->eng->running()) {
+            const double tv = trace ? s->now() : 0;
             Status st = s->eng->advance(done);
+            if (trace) {
+                t_advance += s->now() - tv;
+                n_advance++;
+            }
             if (!st.good()) {
                 std::lock_guard<std::mutex> lk(s->mu);
                 s->fatal = st;
@@ -219,6 +237,9 @@ Status server_submit(Server &s, const HostGraph &query, const gfsm_match_opts *o
     job->opts.flags |= s.opts.flags;
     Status plan_st = (query.mode == s.data->mode) ? plan_for(*s.data, query, job->opts, job->plan)
                                                    : Status::err(GFSM_ERR_ARG, "query and data graph use different modes");
+    // The level model is worked out here, on the submitting thread, so the
+    // worker's admission loop only compares sizes.
+    if (plan_st.good() && job->plan.q > 1) BatchEngine::model_job(*s.data, *job);
     {
         std::lock_guard<std::mutex> lk(s.mu);
         if (s.stop) return s.fatal.good() ? Status::err(GFSM_ERR_ARG, "server stopped") : s.fatal;

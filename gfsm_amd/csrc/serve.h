@@ -41,6 +41,10 @@ struct BatchJob {
     Plan plan;
     uint64_t region_off = 0, region_bytes = 0;
     uint64_t nroots = 0;                 // root candidates of its window
+    // The root window's size and the level model, computed ahead of
+    // admission (by the submitting thread) when `modeled` is set.
+    bool modeled = false;
+    uint64_t rlo = 0, rhi = 0, g1 = 0;
     bool waited_memory = false;          // the server held it back for projected memory (a slot was free)
     bool waited_slot = false;            // ... for a free slot
     double model_bytes = 0;  // projected level bytes before calibration
@@ -70,6 +74,9 @@ public:
     // the data graph's relation statistics, clamped to [min, max] region.
     uint64_t project_bytes(const Plan &plan, uint64_t nroots, uint64_t g1) const;
     double model_bytes(const Plan &plan, uint64_t nroots, uint64_t g1) const;
+    uint64_t project_model(uint32_t q, double model) const;
+    // Fill job's root window, depth-1 candidates and level model.
+    static void model_job(const HostGraph &data, BatchJob &job);
     // Learn the model's error from a finished query (actual / model level bytes).
     void calibrate(double model, const Result &r, uint32_t q);
     double calibration() const { return calib_; }

@@ -204,7 +204,7 @@ Status BatchEngine::init(const BatchShape &shape, uint64_t arena_bytes, char *ar
     // The per-slot step size is step_budget / running (set before each
     // replay), so one step needs at most step_budget / tc + n tiles.
     shape_.step_budget = std::max<uint64_t>(tc, std::min<uint64_t>(shape_.step_budget, (max_tiles - n) * tc));
-    st = lane_.init(max_tiles);
+    st = lane_.init(max_tiles, true);
     if (!st.good()) return st;
     HIP_TRY(hipMalloc(&slots_d_, sizeof(BatchSlot) * n));
     HIP_TRY(hipMalloc(&report_d_, sizeof(SlotReport) * n));
@@ -252,6 +252,7 @@ Status BatchEngine::init(const BatchShape &shape, uint64_t arena_bytes, char *ar
     cfg_.report = report_d_;
     cfg_.final = final_d_;
     cfg_.tile_start = tile_start_d_;
+    cfg_.tile_node = lane_.tile_node;
     cfg_.ctl = ctl_d_;
     cfg_.nslots = n;
     cfg_.items = shape_.items;
@@ -261,9 +262,11 @@ Status BatchEngine::init(const BatchShape &shape, uint64_t arena_bytes, char *ar
     cfg_.g = dg_->view;
     cfg_.ws = lane_.ws;
     grid_ = std::max<uint32_t>(ctx_.num_cus, 1) * std::max<uint32_t>(shape_.wgs_per_cu, 1);
-    // The schedule is fresh between replays: replays run [step, schedule] x K.
-    // One empty step outside capture also sets the step kernel's LDS limit.
+    // The schedule is fresh between replays: replays run [tile index, step,
+    // schedule] x K.  One empty step outside capture also sets the step
+    // kernel's LDS limit.
     HIP_TRY(launch_batch_schedule(cfg_, lane_.stream));
+    HIP_TRY(launch_batch_tile_index(cfg_, lane_.stream));
     HIP_TRY(launch_batch_step(cfg_, grid_, lane_.stream));
     HIP_TRY(launch_batch_schedule(cfg_, lane_.stream));
     HIP_TRY(hipStreamSynchronize(lane_.stream));
@@ -277,7 +280,8 @@ Status BatchEngine::capture()
     HIP_TRY(hipStreamBeginCapture(lane_.stream, hipStreamCaptureModeRelaxed));
     hipError_t e = hipSuccess;
     for (uint32_t i = 0; i < shape_.steps_per_replay && e == hipSuccess; i++) {
-        e = launch_batch_step(cfg_, grid_, lane_.stream);
+        e = launch_batch_tile_index(cfg_, lane_.stream);
+        if (e == hipSuccess) e = launch_batch_step(cfg_, grid_, lane_.stream);
         if (e == hipSuccess) e = launch_batch_schedule(cfg_, lane_.stream);
     }
     hipGraph_t g = nullptr;
@@ -418,6 +422,7 @@ Status BatchEngine::replay()
         HIP_TRY(hipGraphLaunch(exec_, lane_.stream));
     } else {
         for (uint32_t i = 0; i < K; i++) {
+            HIP_TRY(launch_batch_tile_index(cfg_, lane_.stream));
             HIP_TRY(launch_batch_step(cfg_, grid_, lane_.stream));
             HIP_TRY(launch_batch_schedule(cfg_, lane_.stream));
         }

@@ -261,6 +261,7 @@ struct BatchCfg {
     SlotReport *report;           // [nslots]
     SlotFinal *final;             // [nslots], written when a slot finishes
     uint64_t *tile_start;         // [nslots + 1]
+    uint64_t *tile_node;          // [2 * tiles of a step] frontier nodes of every expand tile (k_batch_tile_index)
     BatchCtl *ctl;
     uint32_t nslots;
     uint32_t items;               // candidates per thread (tile = 256 * items)
@@ -278,6 +279,10 @@ struct BatchCfg {
 // memory, so the pair can be captured and replayed.
 hipError_t launch_batch_schedule(const BatchCfg &cfg, hipStream_t s);
 hipError_t launch_batch_step(const BatchCfg &cfg, uint32_t grid, hipStream_t s);
+// Between the scheduler and the step kernel: the frontier nodes of every
+// expand tile of the scheduled step, searched by the whole GPU (one wave per
+// tile), so the step's tiles read two words instead of searching.
+hipError_t launch_batch_tile_index(const BatchCfg &cfg, hipStream_t s);
 
 // Chained-scan self test: out[i] = exclusive prefix of vals (u64), and the
 // indices i with vals[i] != 0 compacted in order into idx; totals in tot.

@@ -1286,7 +1286,10 @@ __global__ __launch_bounds__(kTile) void k_batch_schedule(BatchCfg cfg)
         const uint32_t s = s0 + u;
         if (s < cfg.nslots) {
             cfg.tile_start[s] = base;
-            cfg.slots[s].tile0 = base;
+            BatchSlot &B = cfg.slots[s];
+            B.tile0 = base;
+            // an expand step reads its tiles' frontier nodes from the index
+            if (nt[u] && (B.step == EXPAND_MID || B.step == EXPAND_LAST_COUNT)) B.io.tile_node = cfg.tile_node + 2 * base;
             base += nt[u];
         }
     }
@@ -1314,6 +1317,28 @@ DEV uint32_t slot_search(const uint64_t *__restrict__ start, uint32_t nslots, ui
         hi = (lo + stride < hi) ? lo + stride : hi;
     }
     return lo;
+}
+
+// One wave per tile of the scheduled step: the owning slot, then (expand
+// steps) the nodes holding the tile's first and last candidates, one
+// half-wave each, as k_tile_index does for the single-query path.
+__global__ __launch_bounds__(kTile) void k_batch_tile_index(BatchCfg cfg)
+{
+    const uint64_t total = cfg.ctl->total_tiles;
+    const uint64_t TC = (uint64_t)kTile * cfg.items;
+    const uint32_t lane = lane_id();
+    const uint64_t nwaves = ((uint64_t)gridDim.x * kTile) >> 6;
+    for (uint64_t t = ((uint64_t)blockIdx.x * kTile + threadIdx.x) >> 6; t < total; t += nwaves) {
+        const uint32_t s = slot_search(cfg.tile_start, cfg.nslots, t);
+        const BatchSlot &B = cfg.slots[s];
+        if (B.step != EXPAND_MID && B.step != EXPAND_LAST_COUNT) continue;
+        const uint64_t lt = t - B.tile0;
+        const uint64_t c0 = B.io.g_begin + lt * TC;
+        const uint64_t c1 = (c0 + TC < B.io.g_end) ? c0 + TC : B.io.g_end;
+        const uint64_t key = (lane < 32) ? c0 : c1 - 1;
+        const uint64_t i = frontier_search(gp(B.io.in_off), B.io.S_prev, key, lane & 31);
+        if ((lane & 31) == 0) gp(cfg.tile_node)[2 * t + (lane >> 5)] = i;
+    }
 }
 
 template <uint32_t I>
@@ -1597,6 +1622,13 @@ static hipError_t launch_batch_step_t(const BatchCfg &cfg, uint32_t grid, hipStr
     hipError_t e = allow_lds(&k_batch_step<I>, attr);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_batch_step<I>), dim3(grid), dim3(kTile), lds, s, cfg);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_tile_index(const BatchCfg &cfg, hipStream_t s)
+{
+    if (cfg.nslots == 0 || cfg.nslots > kMaxSlots || !cfg.tile_node) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_batch_tile_index, dim3(2048), dim3(kTile), 0, s, cfg);
     return hipGetLastError();
 }
 

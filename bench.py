@@ -72,13 +72,18 @@ def calibrate(mode):
 def traffic_from_profiles():
     """HBM bytes per expand launch of the timed sample, measured with rocprofv3
     FETCH_SIZE/WRITE_SIZE and scaled by the gather calibration
-    (profiles/expand_config2_traffic.json), or None when not recorded."""
+    (profiles/expand_config2_traffic.json), and where that figure comes from;
+    (None, None) when not recorded.  Counters need a profiler run of their
+    own, so the figure is not measured in the run that prints it."""
     path = os.path.join(ROOT, "profiles", "expand_config2_traffic.json")
     try:
         with open(path) as f:
-            return float(json.load(f)["hbm_bytes_per_launch"])
+            rec = json.load(f)
+        src = (f"rocprofv3 FETCH_SIZE x {rec['fetch_factor']} + WRITE_SIZE x {rec['write_factor']} per expand "
+               f"launch of this sample on commit {rec['commit']} ({rec['source']}); not measured in this run")
+        return float(rec["hbm_bytes_per_launch"]), src
     except (OSError, KeyError, ValueError):
-        return None
+        return None, None
 
 
 def cpu_model():
@@ -467,13 +472,14 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic_from_profiles(),
+            "traffic": None,
             "kernel": "gfsm::k_expand (inner depths) + gfsm::k_expand_bare (last depth)",
             "launches": launches // K,
             "mean_launch_ms": mean_ms,
         },
     }
     ctx.close()
+    line["roofline"]["traffic"], line["roofline"]["traffic_source"] = traffic_from_profiles()
     line["cpu_baseline"] = None if args.no_cpu else cpu_baseline(data, queries, sample, win_s)
     if "grid" in sections:
         line["grid"] = grid_section(max(1, min(K, 3)))

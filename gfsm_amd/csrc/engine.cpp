@@ -921,8 +921,8 @@ Status run_match(Ctx &ctx, const HostGraph &data, const DevGraph &dg, const Plan
     const uint64_t budget = o.mem_budget ? o.mem_budget : ctx.budget;
     Status st = run.carve(fixed ? std::min(budget, fixed->bytes) : budget);
     if (!st.good()) return st;
-    // Capped: inner depths start with an eighth of the cap, then follow the
-    // observed fan-out.  The counting last depth keeps whole launches: it
+    // Capped: inner depths start with cap / kCapStartDiv candidates, then
+    // follow the observed fan-out.  The counting last depth keeps whole launches: it
     // needs no level buffer, and its ranges are trimmed to the cap exactly.
     if (o.max_depth_cand)
         for (uint32_t k = 0; k + 1 < plan.q; k++)

@@ -50,6 +50,8 @@ Status Lane::init(uint64_t max_tiles, bool tile_index)
     HIP_TRY(hipMemset(tot_d, 0, sizeof(Totals)));
     HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&tot_h), sizeof(Totals), hipHostMallocDefault));
     std::memset(tot_h, 0, sizeof(Totals));
+    HIP_TRY(hipHostMalloc(reinterpret_cast<void **>(&kc_h), 2 * sizeof(uint64_t), hipHostMallocCoherent));
+    kc_h[0] = kc_h[1] = 0;
     for (int i = 0; i < 2; i++) HIP_TRY(hipEventCreate(&ev[i]));
     return Status::ok();
 }
@@ -74,6 +76,8 @@ void Lane::destroy()
     tile_node = nullptr;
     (void)hipFree(tot_d);
     if (tot_h) (void)hipHostFree(tot_h);
+    if (kc_h) (void)hipHostFree(kc_h);
+    kc_h = nullptr;
     for (int i = 0; i < 2; i++)
         if (ev[i]) (void)hipEventDestroy(ev[i]);
     for (hipEvent_t e : kev) (void)hipEventDestroy(e);
@@ -535,6 +539,30 @@ struct Runner {
         return Status::ok();
     }
 
+    // Host words of the next root / inner step's {keep, cnt}, armed with a
+    // value no step writes; null when the lane has none (then the totals are
+    // copied).
+    uint64_t *arm_step_totals()
+    {
+        if (!lane.kc_h) return nullptr;
+        lane.kc_h[0] = lane.kc_h[1] = ~0ull;
+        return lane.kc_h;
+    }
+
+    // After a root or non-counting step: wait for it and take its {keep, cnt}
+    // into tot_h, from the host words it wrote or by a copy.
+    Status sync_step_totals(const uint64_t *kc)
+    {
+        if (!kc) return sync_totals();
+        HIP_TRY(hipStreamSynchronize(lane.stream));
+        const uint64_t keep = reinterpret_cast<const volatile uint64_t *>(kc)[0];
+        const uint64_t cnt = reinterpret_cast<const volatile uint64_t *>(kc)[1];
+        if (keep == ~0ull || cnt == ~0ull) return Status::err(GFSM_ERR_DEVICE, "internal: step totals not written");
+        lane.tot_h->keep = keep;
+        lane.tot_h->cnt = cnt;
+        return Status::ok();
+    }
+
     // Per-launch timing (GFSM_PROFILE): an event pair around each matcher
     // launch, read back in batches, so timing adds no host wait between
     // launches beyond those the depth loop makes anyway.
@@ -600,6 +628,7 @@ struct Runner {
             a.io.out_gbeg = gbeg[0];
             a.io.cap = cap[0];
             a.io.tot = lane.tot_d;
+            a.io.host_kc = arm_step_totals();
             a.io.ntiles = (a.io.n + kRootTile - 1) / kRootTile;
             a.nchk = plan.dev.nroot_chk;
             std::memcpy(a.chk_rel, plan.dev.root_chk_rel, sizeof(a.chk_rel));
@@ -615,7 +644,7 @@ struct Runner {
             res.stats.launches++;
             Status st = kend(false);
             if (!st.good()) return st;
-            st = sync_totals();
+            st = sync_step_totals(a.io.host_kc);
             if (!st.good()) return st;
             const uint64_t S0 = lane.tot_h->keep, G1 = lane.tot_h->cnt;
             res.stats.level[0] += S0;
@@ -690,6 +719,7 @@ struct Runner {
                 io.cap = rows_cap;
             }
             io.tot = lane.tot_d;
+            io.host_kc = (kind == EXPAND_MID || kind == EXPAND_LAST_ROWS) ? arm_step_totals() : nullptr;
             io.ntiles = (e - c + tc - 1) / tc;
             a.ws = lane.ws;
             // Launch-shape checks: the workspace holds every tile, the frontier
@@ -728,7 +758,7 @@ struct Runner {
                 c = e;
                 continue;
             }
-            st = sync_totals();
+            st = sync_step_totals(io.host_kc);
             if (!st.good()) return st;
             const uint64_t keep = lane.tot_h->keep;
             if (keep > e - c) return Status::err(GFSM_ERR_DEVICE, "internal: more nodes kept than candidates");
@@ -1074,6 +1104,7 @@ Status match_noalloc(Ctx &ctx, const HostGraph &data, const HostGraph &query, co
     L.ev[0] = ctx.lane.ev[0];
     L.ev[1] = ctx.lane.ev[1];
     L.tot_h = ctx.lane.tot_h;
+    L.kc_h = ctx.lane.kc_h;
     L.ws.ctl = reinterpret_cast<uint32_t *>(p);
     L.tot_d = reinterpret_cast<Totals *>(p + 256);
     L.ws.wk = reinterpret_cast<uint64_t *>(p + 512);

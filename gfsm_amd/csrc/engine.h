@@ -46,6 +46,10 @@ struct Lane {
     ScanWS ws{};
     Totals *tot_d = nullptr;
     Totals *tot_h = nullptr;  // pinned
+    // Fine-grained host words {keep, cnt} that root and inner-depth steps
+    // write beside their device totals: the depth loop reads a step's totals
+    // after a stream wait, with no copy launch.
+    uint64_t *kc_h = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr};
     std::vector<hipEvent_t> kev;  // begin/end pairs of timed launches (grown on demand)
     uint64_t launches = 0;  // chained-scan launches issued on this lane

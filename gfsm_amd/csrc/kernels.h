@@ -89,6 +89,8 @@ struct StepIO {
     uint32_t pad;
     const uint64_t *tile_node; // [2*ntiles] first and last frontier node of each tile (k_tile_index),
                                // or null: each tile searches in_off itself
+    uint64_t *host_kc;         // or null: fine-grained host words that also receive {keep, cnt},
+                               // so the host reads a step's totals without a copy
 };
 constexpr uint32_t kDenseChk = 4;  // checked relations per depth with a dense-table slot (DepthPlanDev)
 
@@ -104,6 +106,7 @@ struct RootIO {
     uint64_t cap;
     Totals *tot;
     uint64_t ntiles;
+    uint64_t *host_kc;         // as StepIO::host_kc
 };
 
 struct ExpandArgs {

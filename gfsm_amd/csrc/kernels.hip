@@ -457,6 +457,10 @@ DEV void root_tile(const DevGraphView &g, const RootIO &io, uint32_t nchk, CP ch
         gp(io.tot)->keep = rk;
         gp(io.tot)->cnt = rc;
         if (rk <= io.cap) gp(io.out_off)[rk] = rc;
+        if (io.host_kc) {
+            gp(io.host_kc)[0] = rk;
+            gp(io.host_kc)[1] = rc;
+        }
     }
 }
 
@@ -893,6 +897,10 @@ DEV void expand_tile(const DevGraphView &g, const DP &dp, CP col, CP dcol,
             if (lt == io.ntiles - 1 && tid == 0) {
                 tot->keep = sm.tot[0];
                 tot->cnt = sm.tot[1];
+                if (io.host_kc) {
+                    gp(io.host_kc)[0] = sm.tot[0];
+                    gp(io.host_kc)[1] = sm.tot[1];
+                }
                 if (KIND == EXPAND_MID && sm.tot[0] <= io.cap) out_off[sm.tot[0]] = sm.tot[1];
             }
         } else {

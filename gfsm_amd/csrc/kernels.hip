@@ -48,6 +48,7 @@ DEV gptr_t<T> gp(T *p)
 constexpr uint32_t kWaves = kTile / 64;
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));  // one 16-byte store
 constexpr uint32_t kNodeLk = 4;  // node-setup lookups a thread keeps in flight
+constexpr uint32_t kStageLd = 4; // row words a thread loads before storing them to LDS
 
 // ---------------------------------------------------------------------------
 // wave64 primitives
@@ -226,7 +227,13 @@ DEV uint64_t lookback(const uint64_t *words, uint32_t ep, uint64_t t, uint64_t t
             const uint64_t xm = __ballot(st == 0u);
             const uint64_t upto = pm ? (pm ^ (pm - 1)) : ~0ull;
             if (!(xm & upto)) break;
-            __builtin_amdgcn_s_sleep(1);
+            // The longest back-off (about 8,000 clocks) between polls: each
+            // poll is an agent-scope read of up to 64 words that goes past
+            // the L2, and a waiting tile holds up no other tile, since its
+            // aggregate is already published (measured: polling after
+            // s_sleep(1) made the power-law sample 5% slower; a short first
+            // retry and then the longest gained nothing over the longest alone).
+            __builtin_amdgcn_s_sleep(127);
         }
         const uint64_t upto = pm ? (pm ^ (pm - 1)) : ~0ull;
         const uint64_t v = ((upto >> lane) & 1ull) ? (w & kValueMask) : 0ull;
@@ -399,6 +406,14 @@ DEV void flush_sums(uint64_t cnt, uint64_t h0, uint64_t h1, Totals *tot_ptr, uin
     }
 }
 
+// One per-node lookup of a tile's staging (a checked ancestor's segment or
+// the next generator's): the row column whose vertex is looked up (~0u: none,
+// the check is the candidate's own), its relation and its dense table.
+struct NodeLookup {
+    uint32_t col, rel;
+    const void *tab;
+};
+
 // Statically allocated LDS of one tile loop.
 template <uint32_t I>
 struct TileShared {
@@ -408,6 +423,7 @@ struct TileShared {
     uint64_t red[3][kWaves];
     uint32_t slot;
     uint32_t next;  // prefetched tile ticket
+    NodeLookup lk[GFSM_MAXCHK + 1];  // the depth's lookups, read per lane from LDS
 };
 
 // ---------------------------------------------------------------------------
@@ -585,6 +601,25 @@ DEV void expand_tile(const DevGraphView &g, const DP &dp, CP col, CP dcol,
     const auto out_rows = gp(io.rows);
     const auto tot = gp(io.tot);
 
+    const uint32_t nchk = BARE ? 0u : dp.nchk;
+    const uint32_t need_nxt = (KIND == EXPAND_MID) ? io.need_nxt : 0u;
+    // The depth's lookups, one LDS entry each, so the staging loop below
+    // selects its lookup per lane from LDS rather than indexing the plan
+    // (kernel arguments or global memory) with a per-lane index.
+    if (!BARE && tid < nchk + need_nxt) {
+        NodeLookup e;
+        if (tid < nchk) {
+            const uint32_t dist = dp.chk_dist[tid];
+            e.col = dist ? d - dist : ~0u;  // self checks: per candidate
+            e.rel = dp.chk_rel[tid];
+            e.tab = tid < kDenseChk ? dp.chk_seg[tid] : nullptr;
+        } else {
+            e.col = d + 1 - dp.nxt_dist;
+            e.rel = dp.nxt_rel;
+            e.tab = dp.nxt_seg;
+        }
+        sh.lk[tid] = e;
+    }
     // Frontier nodes covering [c0, c1): offsets are strictly increasing.
     if (tid < NW) s_bits[tid] = 0;
     if (io.tile_node) {
@@ -599,8 +634,6 @@ DEV void expand_tile(const DevGraphView &g, const DP &dp, CP col, CP dcol,
     if (KIND == EXPAND_LAST_COUNT && tickets) ticket_prefetch(ws, &sh.next);  // no look-back in this kind
     const uint64_t ilo = sh.node[0];
     const uint32_t nf = (uint32_t)(sh.node[1] - ilo + 1);
-    const uint32_t nchk = BARE ? 0u : dp.nchk;
-    const uint32_t need_nxt = (KIND == EXPAND_MID) ? io.need_nxt : 0u;
     // The rows come first: every candidate reads its node's row (d words,
     // twice on inner depths), but a node's segments save lookups only for
     // its candidates after the first.
@@ -619,15 +652,33 @@ DEV void expand_tile(const DevGraphView &g, const DP &dp, CP col, CP dcol,
     uint64_t *s_part = reinterpret_cast<uint64_t *>(s_pool + pool_used);         // [nf][2]
     const auto in_rows = gp(io.in_vec) + ilo * d;       // the tile's node rows, contiguous
 
-    if (staged)
-        for (uint32_t u = tid; u < nf * d; u += T) s_rows[u] = in_rows[u];
-    // Per frontier node: start bit, generator base.
+    // The rows' words, kStageLd loads in flight per thread before their LDS
+    // stores (one load, wait, store per word would chain round trips).
+    if (staged) {
+        const uint32_t nw = nf * d;
+        for (uint32_t u0 = tid; u0 < nw; u0 += T * kStageLd) {
+            uint32_t v[kStageLd];
+#pragma unroll
+            for (uint32_t b = 0; b < kStageLd; b++) {
+                const uint32_t u = u0 + b * T;
+                v[b] = u < nw ? in_rows[u] : 0u;
+            }
+#pragma unroll
+            for (uint32_t b = 0; b < kStageLd; b++) {
+                const uint32_t u = u0 + b * T;
+                if (u < nw) s_rows[u] = v[b];
+            }
+        }
+    }
+    // Per frontier node: start bit, generator base (both words loaded before
+    // either is used).
     for (uint32_t j = tid; j < nf; j += T) {
         const uint64_t i = ilo + j;
         const uint64_t o = in_off[i];
+        const uint32_t gbv = in_gbeg[i];
         const uint32_t first = (o > c0) ? (uint32_t)(o - c0) : 0u;
         atomicOr((unsigned long long *)&s_bits[first >> 6], 1ull << (first & 63));
-        s_seg[j] = in_gbeg[i] + (uint32_t)(c0 - o);
+        s_seg[j] = gbv + (uint32_t)(c0 - o);
         if (part_ok) {
             uint64_t p0 = 0, p1 = 0;
             for (uint32_t c = 0; c < io.q; c++) {
@@ -664,12 +715,12 @@ DEV void expand_tile(const DevGraphView &g, const DP &dp, CP col, CP dcol,
                     // row column of the vertex whose segment is looked up
                     // (a check's ancestor at distance chk_dist, or the next
                     // generator's at distance nxt_dist - 1 from this depth)
-                    act[k] = (r < nchk) ? dp.chk_dist[r] != 0 : true;  // self checks: per candidate
-                    const uint32_t col = (r < nchk) ? d - dp.chk_dist[r] : d + 1 - dp.nxt_dist;
-                    rel[k] = (r < nchk) ? dp.chk_rel[r] : dp.nxt_rel;
-                    dst[k] = (r < nchk) ? r * nf + j : nchk * nf + j;  // s_chk[r][j] or s_nxt[j]
-                    tab[k] = (r < nchk) ? (r < kDenseChk ? dp.chk_seg[r] : nullptr) : dp.nxt_seg;
-                    if (act[k]) x[k] = in_rows[(uint64_t)j * d + col];
+                    const NodeLookup e = sh.lk[r];
+                    act[k] = e.col != ~0u;
+                    rel[k] = e.rel;
+                    dst[k] = r * nf + j;  // s_chk[r][j], or s_nxt[j] for r == nchk
+                    tab[k] = e.tab;
+                    if (act[k]) x[k] = in_rows[(uint64_t)j * d + e.col];
                 }
             }
             // dense tables where the relation has one, hash lookups for the rest

@@ -568,6 +568,18 @@ DEV uint64_t frontier_search(P off, uint64_t S, uint64_t key, uint32_t sub)
     return lo;
 }
 
+// Column c of the node row at `row` (c < d), or `own` (the candidate itself)
+// when c == d.  The row word is read for every lane, at column 0 where it is
+// not wanted, and the value is selected afterwards: a select between the two
+// sources' addresses would put the candidates' register array in scratch
+// memory and make every such read a flat load.
+template <class RP>
+DEV uint32_t pick_row_or(RP rows, uint32_t row, uint32_t c, uint32_t d, uint32_t own)
+{
+    const uint32_t v = rows[row + (c < d ? c : 0u)];
+    return c < d ? v : own;
+}
+
 // Expand tile `lt` of a step (global tile gt; the step's scan starts at gt0).
 // `dyn` is the dynamic LDS: the node-start bitmap, the per-position generator
 // base and a pool of pool_words words that stages, per frontier node of the
@@ -796,7 +808,7 @@ DEV void expand_tile(const DevGraphView &g, const DP &dp, CP col, CP dcol,
             } else {
                 uint32_t x[I];
 #pragma unroll
-                for (uint32_t k = 0; k < I; k++) x[k] = dist ? rows[jj[k] * d + (d - dist)] : w[k];
+                for (uint32_t k = 0; k < I; k++) x[k] = pick_row_or(rows, jj[k] * d, dist ? d - dist : d, d, w[k]);
                 const void *ct = r < kDenseChk ? dp.chk_seg[r] : nullptr;
                 if (ct) {
                     const auto tab = gp(reinterpret_cast<const uint64_t *>(ct));
@@ -823,14 +835,15 @@ DEV void expand_tile(const DevGraphView &g, const DP &dp, CP col, CP dcol,
                 const auto tab = gp(reinterpret_cast<const uint64_t *>(dp.nxt_seg));
 #pragma unroll
                 for (uint32_t k = 0; k < I; k++) {
-                    const uint32_t x = need_nxt ? rows[jj[k] * d + (d + 1 - dp.nxt_dist)] : w[k];
+                    const uint32_t x = pick_row_or(rows, jj[k] * d, need_nxt ? d + 1 - dp.nxt_dist : d, d, w[k]);
                     const uint64_t v = keep[k] ? tab[x] : 0ull;
                     ns[k] = make_uint2((uint32_t)v, (uint32_t)(v >> 32));
                 }
             } else {
                 uint32_t x[I];
 #pragma unroll
-                for (uint32_t k = 0; k < I; k++) x[k] = need_nxt ? rows[jj[k] * d + (d + 1 - dp.nxt_dist)] : w[k];
+                for (uint32_t k = 0; k < I; k++)
+                    x[k] = pick_row_or(rows, jj[k] * d, need_nxt ? d + 1 - dp.nxt_dist : d, d, w[k]);
                 seg_lookup_n<I>(g, x, dp.nxt_rel, keep, ns);
             }
 #pragma unroll
@@ -974,7 +987,7 @@ DEV void expand_tile(const DevGraphView &g, const DP &dp, CP col, CP dcol,
                 uint64_t x0 = 0, x1 = 0;
                 for (uint32_t c = 0; c < io.q; c++) {
                     const uint32_t dc = dcol[c];
-                    const uint32_t val = (dc == d) ? w[k] : rows[jj[k] * d + dc];
+                    const uint32_t val = pick_row_or(rows, jj[k] * d, dc, d, w[k]);
                     const uint64_t xv = (uint64_t)val | ((uint64_t)c << 32);
                     x0 += mix64(0x9E3779B97F4A7C15ull ^ xv);
                     x1 += mix64(0xD1B54A32D192ED03ull ^ xv);

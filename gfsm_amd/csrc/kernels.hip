@@ -795,8 +795,11 @@ DEV void expand_tile(const DevGraphView &g, const DP &dp, CP col, CP dcol,
         // Phase 2: injectivity against the node's placed vertices, one placed
         // column at a time for all items, so I reads are in flight together.
         for (uint32_t kk = 0; kk < d; kk++) {
+            uint32_t pv[I];
 #pragma unroll
-            for (uint32_t k = 0; k < I; k++) keep[k] &= (rows[jj[k] * d + kk] != w[k]);
+            for (uint32_t k = 0; k < I; k++) pv[k] = rows[jj[k] * d + kk];
+#pragma unroll
+            for (uint32_t k = 0; k < I; k++) keep[k] &= pv[k] != w[k];
         }
         // Phase 3: membership in every checked segment, I searches in lockstep.
         for (uint32_t r = 0; r < nchk; r++) {

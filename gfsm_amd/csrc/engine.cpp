@@ -382,8 +382,23 @@ constexpr uint64_t kCapStartDiv = 64;
 // Steps of at least this many tiles get their frontier nodes from k_tile_index.
 constexpr uint64_t kTileIndexMin = 64;
 
-// Candidates per thread of the lean last-depth kernel (its registers fit 64).
-constexpr uint32_t kBareItems = 4;
+// Candidates per thread of the lean last-depth kernel (its registers fit 64):
+// 8 where the tile's node rows are expected to fit its staging pool, else 4.
+constexpr uint32_t kBareItems = 8;
+constexpr uint32_t kBareItemsNarrow = 4;
+
+// Width of the lean kernel for a last depth of `cands` candidates over
+// `nodes` frontier nodes of `d` placed vertices: 8 candidates per thread when
+// an 8-wide tile's nodes (2048 / mean fan-out + 1) keep their rows within the
+// pool, else 4, whose tiles cover half as many nodes (measured: 8 is 3% faster
+// on the power-law sample; on the road-like grid, fan-out ~4, the rows no
+// longer fit at 8 and it is 4% slower).
+static uint32_t bare_items(uint64_t cands, uint64_t nodes, uint32_t d)
+{
+    const double fan = nodes ? (double)cands / (double)nodes : 1.0;
+    const double tile_nodes = (double)(kTile * kBareItems) / std::max(fan, 1.0) + 1.0;
+    return tile_nodes * d <= (double)kBarePoolWords ? kBareItems : kBareItemsNarrow;
+}
 
 // One plain match in flight: level buffers carved from the arena, the lane,
 // and the depth-first chunk recursion.  Level k (nodes with k+1 placed
@@ -672,7 +687,8 @@ struct Runner {
         const uint32_t kind =
             last ? (rows_mode ? EXPAND_LAST_ROWS : (bare ? EXPAND_LAST_BARE : EXPAND_LAST_COUNT)) : EXPAND_MID;
         const uint64_t tile_limit = launch_limit();
-        const uint32_t items = bare ? std::min(pick_items(EXPAND_LAST_COUNT, d), kBareItems) : pick_items(kind, d);
+        const uint32_t items =
+            bare ? std::min(pick_items(EXPAND_LAST_COUNT, d), bare_items(g1 - g0, S_prev, d)) : pick_items(kind, d);
         const uint32_t pool_words = bare ? std::min(ctx.tune.pool_words, kBarePoolWords) : ctx.tune.pool_words;
         const uint32_t grid_mul = bare ? 2 : 1;  // twice the resident workgroups
         const uint64_t tc = (uint64_t)kTile * items;

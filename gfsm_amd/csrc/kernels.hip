@@ -756,9 +756,9 @@ DEV void expand_tile(const DevGraphView &g, const DP &dp, CP col, CP dcol,
     // of item k) and the set bits before each of them.
     uint32_t wpre[I];
     uint64_t wbits[I];
-    if constexpr (I > 4) {
+    if constexpr (I > 4 && !BARE) {
         // Every wave sums all words: at 8 candidates per thread the scalar
-        // form below costs the inner kernel more in register allocation than
+        // form below costs the general kernel more in register allocation than
         // it saves (measured on the road-like grid: 52.7-54.5 against 51.7 ms).
         uint32_t acc = 0;
 #pragma unroll

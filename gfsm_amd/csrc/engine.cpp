@@ -564,14 +564,31 @@ struct Runner {
         return lane.kc_h;
     }
 
-    // After a root or non-counting step: wait for it and take its {keep, cnt}
-    // into tot_h, from the host words it wrote or by a copy.
-    Status sync_step_totals(const uint64_t *kc)
+    // After a root or non-counting step: take its {keep, cnt} into tot_h,
+    // from the host words it wrote or by a copy.  With `early`, the host
+    // takes them as soon as the step's last tile has published them, while
+    // the step's other workgroups may still be finishing: only for steps whose
+    // next use is a launch on the same stream (stream order makes it wait).
+    // The stream is queried now and then, so a step that ends without
+    // publishing is an error, not a hang.
+    Status sync_step_totals(const uint64_t *kc, bool early = false)
     {
         if (!kc) return sync_totals();
-        HIP_TRY(hipStreamSynchronize(lane.stream));
-        const uint64_t keep = reinterpret_cast<const volatile uint64_t *>(kc)[0];
-        const uint64_t cnt = reinterpret_cast<const volatile uint64_t *>(kc)[1];
+        const volatile uint64_t *w = reinterpret_cast<const volatile uint64_t *>(kc);
+        if (early) {
+            for (uint32_t i = 1; w[0] == ~0ull; i++) {
+                if ((i & 255) == 0) {
+                    const hipError_t q = hipStreamQuery(lane.stream);
+                    if (q == hipSuccess) break;  // ended: the words are final
+                    if (q != hipErrorNotReady) return Status::err(GFSM_ERR_DEVICE, hipGetErrorString(q));
+                }
+                __builtin_ia32_pause();
+            }
+        } else {
+            HIP_TRY(hipStreamSynchronize(lane.stream));
+        }
+        const uint64_t keep = w[0];
+        const uint64_t cnt = w[1];
         if (keep == ~0ull || cnt == ~0ull) return Status::err(GFSM_ERR_DEVICE, "internal: step totals not written");
         lane.tot_h->keep = keep;
         lane.tot_h->cnt = cnt;
@@ -659,7 +676,7 @@ struct Runner {
             res.stats.launches++;
             Status st = kend(false);
             if (!st.good()) return st;
-            st = sync_step_totals(a.io.host_kc);
+            st = sync_step_totals(a.io.host_kc, true);
             if (!st.good()) return st;
             const uint64_t S0 = lane.tot_h->keep, G1 = lane.tot_h->cnt;
             res.stats.level[0] += S0;
@@ -774,7 +791,7 @@ struct Runner {
                 c = e;
                 continue;
             }
-            st = sync_step_totals(io.host_kc);
+            st = sync_step_totals(io.host_kc, kind == EXPAND_MID);
             if (!st.good()) return st;
             const uint64_t keep = lane.tot_h->keep;
             if (keep > e - c) return Status::err(GFSM_ERR_DEVICE, "internal: more nodes kept than candidates");

@@ -430,6 +430,14 @@ struct TileShared {
 // root level (hnInit, HnGPU.cu:76-109)
 // ---------------------------------------------------------------------------
 
+// A step's {keep, cnt} in the host's words: cnt first, then keep behind a
+// system-scope fence, so a host that sees keep sees cnt too (it polls keep).
+DEV void publish_kc(uint64_t *kc, uint64_t keep, uint64_t cnt)
+{
+    __hip_atomic_store(kc + 1, cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(kc, keep, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // Root tile `lt` of a root step (global tile gt, the step's scan starting at
 // gt0): keep candidates that carry every self-loop relation of the root.
 template <uint32_t I, class CP>
@@ -473,10 +481,7 @@ DEV void root_tile(const DevGraphView &g, const RootIO &io, uint32_t nchk, CP ch
         gp(io.tot)->keep = rk;
         gp(io.tot)->cnt = rc;
         if (rk <= io.cap) gp(io.out_off)[rk] = rc;
-        if (io.host_kc) {
-            gp(io.host_kc)[0] = rk;
-            gp(io.host_kc)[1] = rc;
-        }
+        if (io.host_kc) publish_kc(io.host_kc, rk, rc);
     }
 }
 
@@ -984,10 +989,7 @@ DEV void expand_tile(const DevGraphView &g, const DP &dp, CP col, CP dcol,
             if (lt == io.ntiles - 1 && tid == 0) {
                 tot->keep = sm.tot[0];
                 tot->cnt = sm.tot[1];
-                if (io.host_kc) {
-                    gp(io.host_kc)[0] = sm.tot[0];
-                    gp(io.host_kc)[1] = sm.tot[1];
-                }
+                if (io.host_kc) publish_kc(io.host_kc, sm.tot[0], sm.tot[1]);
                 if (KIND == EXPAND_MID && sm.tot[0] <= io.cap) out_off[sm.tot[0]] = sm.tot[1];
             }
         } else {

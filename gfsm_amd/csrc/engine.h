@@ -76,7 +76,7 @@ struct Tuning {
     uint32_t pool_words = kPoolWords;
     uint32_t wgs_per_cu = 4;
     uint32_t batch_items = 8;
-    uint32_t batch_pool_words = 14336;
+    uint32_t batch_pool_words = kPoolWords;  // four workgroups per CU, as the step kernel's registers allow
     uint32_t batch_wgs_per_cu = 4;
     uint32_t batch_slots = 256;
     uint32_t batch_steps = 4;  // steps per captured replay of gfsm_match_batch

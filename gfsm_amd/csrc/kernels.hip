@@ -1438,8 +1438,12 @@ __global__ __launch_bounds__(kTile) void k_batch_tile_index(BatchCfg cfg)
     }
 }
 
+// Four waves per SIMD, as k_expand: at 8 candidates per thread the step
+// kernel would take 207 VGPRs (two waves per SIMD); capped at 128 it spills,
+// but with a 7,680-word pool four workgroups fit a CU and the batched call
+// runs 15% faster (measured: 48.0 against 56.6 ms on the 256-query batch).
 template <uint32_t I>
-__global__ __launch_bounds__(kTile) void k_batch_step(BatchCfg cfg)
+__global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(4))) void k_batch_step(BatchCfg cfg)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ TileShared<I> sh;

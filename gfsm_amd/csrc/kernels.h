@@ -245,6 +245,9 @@ struct BatchCtl {
                                   // between replays; 0 = max_step_cand)
     uint32_t running;             // slots in SLOT_NEW or SLOT_RUN after scheduling
     uint32_t n_admit;             // admission entries waiting (set by the host, cleared by the scheduler)
+    uint32_t sched_done;          // scheduler workgroups finished (the last one lays out the tiles
+                                  // and resets it)
+    uint32_t pad;
 };
 
 // One admission, written by the host into a contiguous queue between

@@ -1132,11 +1132,12 @@ DEV uint64_t capped_chunk(const BatchSlot &B, uint32_t d, uint64_t n, uint64_t c
 }
 
 // Consume the finished step of slot B (depth-first chunk walk), then
-// schedule its next step.  Returns the new step's tile count (0 when the slot
+// schedule its next step.  B may be a copy of the slot; `tot` is the slot's
+// own totals word block, which the step writes.  Returns the new step's tile count (0 when the slot
 // has no step).  Mirrors the single-query driver's recursion: a level that
 // overflows its buffer is retried on half the range; otherwise the chunk is
 // re-estimated from the observed selectivity.
-DEV uint32_t schedule_slot(const BatchCfg &cfg, BatchSlot &B)
+DEV uint32_t schedule_slot(const BatchCfg &cfg, BatchSlot &B, Totals *tot)
 {
     const uint64_t TC = (uint64_t)kTile * cfg.items;
     const uint64_t by_val = kValueMask / umax64(cfg.max_degree, 1);
@@ -1266,7 +1267,7 @@ DEV uint32_t schedule_slot(const BatchCfg &cfg, BatchSlot &B)
             io.cap = B.cap[d];
             io.need_nxt = dp.nxt_dist > 1 ? 1u : 0u;
         }
-        io.tot = &B.tot;
+        io.tot = tot;
         io.ntiles = (e - b + TC - 1) / TC;
         io.d = (uint32_t)d;
         io.q = q;
@@ -1291,7 +1292,7 @@ DEV uint32_t schedule_slot(const BatchCfg &cfg, BatchSlot &B)
         r.out_off = B.off[0];
         r.out_gbeg = B.gbeg[0];
         r.cap = B.cap[0];
-        r.tot = &B.tot;
+        r.tot = tot;
         r.ntiles = (e - b + TC - 1) / TC;
         B.rio = r;
         B.step = STEP_ROOT;
@@ -1307,64 +1308,125 @@ DEV uint32_t schedule_slot(const BatchCfg &cfg, BatchSlot &B)
     return ntiles;
 }
 
-// One workgroup: install queued admissions, schedule every slot, then lay
-// the slots' tiles end to end.
+// A slot's words, `n` of them, copied by the 64 lanes of one wave with
+// every lane's loads issued before its stores.
+template <uint32_t N, class SP, class DP>
+DEV void wave_copy_words(DP dst, SP src, uint32_t lane)
+{
+    constexpr uint32_t kBatch = 16;
+    for (uint32_t w0 = 0; w0 < N; w0 += 64 * kBatch) {
+        uint32_t v[kBatch];
+#pragma unroll
+        for (uint32_t i = 0; i < kBatch; i++) {
+            const uint32_t w = w0 + i * 64 + lane;
+            v[i] = w < N ? src[w] : 0u;
+        }
+#pragma unroll
+        for (uint32_t i = 0; i < kBatch; i++) {
+            const uint32_t w = w0 + i * 64 + lane;
+            if (w < N) dst[w] = v[i];
+        }
+    }
+}
+
+DEV void wave_sync_lds()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One wave per slot: install the slot's queued admission, copy the slot into
+// LDS (its words in flight together), schedule it there -- the scheduling
+// logic's chain of dependent reads and writes then runs at LDS latency, not
+// at that of global memory -- and write it back.  The last workgroup to
+// finish lays the slots' tiles end to end.
+constexpr uint32_t kSlotWords = sizeof(BatchSlot) / 4;
+constexpr uint32_t kPlanWords = sizeof(QueryPlanDev) / 4;
+// tile_start[s] between a slot's scheduling and the layout: its tile count,
+// whether it is running, and whether its step reads the tile index.
+constexpr uint64_t kSchedRun = 1ull << 63, kSchedIndexed = 1ull << 62;
+
 __global__ __launch_bounds__(kTile) void k_batch_schedule(BatchCfg cfg)
 {
+    static_assert(sizeof(BatchSlot) % 8 == 0 && sizeof(QueryPlanDev) % 4 == 0, "word copies");
+    __shared__ __align__(16) uint64_t s_slot[kWaves][kSlotWords / 2];
     __shared__ uint32_t s_wt[kWaves], s_wr[kWaves];
+    __shared__ uint32_t s_last;
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
-    const uint32_t n_admit = cfg.ctl->n_admit;
-    if (n_admit) {
-        static_assert(sizeof(BatchSlot) % 4 == 0 && sizeof(QueryPlanDev) % 4 == 0, "word copies");
-        // All entries' words at once, striped over the workgroup, so the
-        // copies' loads overlap instead of one entry after another.
-        constexpr uint32_t SW = sizeof(BatchSlot) / 4, PW = sizeof(QueryPlanDev) / 4;
+    const uint32_t s = blockIdx.x * kWaves + wave;
+    const auto slots = gp(cfg.slots);
+    const auto tile_start = gp(cfg.tile_start);
+    if (s < cfg.nslots) {
+        uint32_t *lw = reinterpret_cast<uint32_t *>(s_slot[wave]);
+        // this slot's queued admission, if any (its entry is found by a ballot)
+        const uint32_t n_admit = gp(cfg.ctl)->n_admit;
         const uint32_t ne = n_admit < cfg.nslots ? n_admit : cfg.nslots;
-        for (uint32_t i = tid; i < ne * (SW + PW); i += blockDim.x) {
-            const uint32_t e = i / (SW + PW), w = i - e * (SW + PW);
-            const AdmitEntry &a = cfg.admit[e];
-            const uint32_t s = a.slot;
-            if (s >= cfg.nslots) continue;
-            if (w < SW)
-                reinterpret_cast<uint32_t *>(cfg.slots + s)[w] = reinterpret_cast<const uint32_t *>(&a.s)[w];
-            else
-                reinterpret_cast<uint32_t *>(cfg.plans + s)[w - SW] = reinterpret_cast<const uint32_t *>(&a.plan)[w - SW];
-        }
-        // The copies are read back below by other threads of this workgroup.
-        __threadfence();
-        __syncthreads();
-        __threadfence();
-        if (tid == 0) cfg.ctl->n_admit = 0;
-    }
-    const uint32_t per = (cfg.nslots + kTile - 1) / kTile;  // <= kMaxSlots / kTile
-    const uint32_t s0 = tid * per;
-    uint32_t nt[kMaxSlots / kTile];
-    uint32_t my_tiles = 0, my_run = 0;
-    for (uint32_t u = 0; u < per; u++) {
-        nt[u] = 0;
-        const uint32_t s = s0 + u;
-        if (s < cfg.nslots) {
-            BatchSlot &B = cfg.slots[s];
-            const uint32_t before = B.state;
-            nt[u] = schedule_slot(cfg, B);
-            if (B.state >= SLOT_DONE && before < SLOT_DONE) {  // just finished: its statistics
-                SlotFinal &f = cfg.final[s];
-                f.overflows = B.overflows;
-                for (uint32_t k = 0; k < GFSM_MAXQ; k++) {
-                    f.level[k] = B.level[k];
-                    f.cand[k] = B.cand[k];
-                }
+        const auto admit = gp(cfg.admit);
+        uint32_t found = ~0u;
+        for (uint32_t e0 = 0; e0 < ne; e0 += 64) {
+            const uint32_t e = e0 + lane;
+            const uint64_t m = __ballot(e < ne && admit[e].slot == s);
+            if (m) {
+                found = e0 + (uint32_t)__builtin_ctzll(m);
+                break;
             }
-            my_tiles += nt[u];
-            my_run += (B.state == SLOT_RUN) ? 1u : 0u;
-            SlotReport rp;
+        }
+        if (found != ~0u) {
+            wave_copy_words<kSlotWords>(lw, reinterpret_cast<gptr_t<const uint32_t>>(&admit[found].s), lane);
+            wave_copy_words<kPlanWords>(reinterpret_cast<gptr_t<uint32_t>>(gp(cfg.plans) + s),
+                                        reinterpret_cast<gptr_t<const uint32_t>>(&admit[found].plan), lane);
+        } else {
+            wave_copy_words<kSlotWords>(lw, reinterpret_cast<gptr_t<const uint32_t>>(slots + s), lane);
+        }
+        wave_sync_lds();
+        BatchSlot &B = *reinterpret_cast<BatchSlot *>(lw);
+        uint32_t fin = 0;
+        if (lane == 0) {
+            const uint32_t before = B.state;
+            const uint32_t nt = schedule_slot(cfg, B, &cfg.slots[s].tot);
+            fin = (B.state >= SLOT_DONE && before < SLOT_DONE) ? 1u : 0u;  // just finished: its statistics
+            SlotReport &rp = cfg.report[s];
             rp.state = B.state;
             rp.steps = B.steps;
             rp.count = B.count;
             rp.dg0 = B.dg0;
             rp.dg1 = B.dg1;
-            cfg.report[s] = rp;
+            const bool indexed = nt && (B.step == EXPAND_MID || B.step == EXPAND_LAST_COUNT);
+            tile_start[s] = (uint64_t)nt | (B.state == SLOT_RUN ? kSchedRun : 0ull) | (indexed ? kSchedIndexed : 0ull);
         }
+        fin = __builtin_amdgcn_readfirstlane(fin);
+        wave_sync_lds();
+        if (fin) {
+            SlotFinal &f = cfg.final[s];
+            if (lane == 0) f.overflows = B.overflows;
+            if (lane < GFSM_MAXQ) {
+                f.level[lane] = B.level[lane];
+                f.cand[lane] = B.cand[lane];
+            }
+        }
+        wave_copy_words<kSlotWords>(reinterpret_cast<gptr_t<uint32_t>>(slots + s), lw, lane);
+    }
+    // The last workgroup to finish lays the tiles out: every other one has
+    // written its slots (released by its count, acquired here).
+    __syncthreads();
+    if (tid == 0) {
+        const uint32_t done =
+            __hip_atomic_fetch_add(&gp(cfg.ctl)->sched_done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        s_last = (done + 1 == gridDim.x) ? 1u : 0u;
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    const uint32_t per = (cfg.nslots + kTile - 1) / kTile;  // <= kMaxSlots / kTile
+    const uint32_t s0 = tid * per;
+    uint64_t packed[kMaxSlots / kTile];
+    uint32_t my_tiles = 0, my_run = 0;
+#pragma unroll
+    for (uint32_t u = 0; u < kMaxSlots / kTile; u++) {
+        packed[u] = (u < per && s0 + u < cfg.nslots) ? tile_start[s0 + u] : 0ull;
+        my_tiles += (uint32_t)packed[u];
+        my_run += (packed[u] & kSchedRun) ? 1u : 0u;
     }
     const uint32_t inc = wave_incl_scan(my_tiles);
     const uint32_t rinc = wave_incl_scan(my_run);
@@ -1379,21 +1441,24 @@ __global__ __launch_bounds__(kTile) void k_batch_schedule(BatchCfg cfg)
         total += s_wt[w];
         running += s_wr[w];
     }
-    for (uint32_t u = 0; u < per; u++) {
-        const uint32_t s = s0 + u;
-        if (s < cfg.nslots) {
-            cfg.tile_start[s] = base;
-            BatchSlot &B = cfg.slots[s];
-            B.tile0 = base;
+#pragma unroll
+    for (uint32_t u = 0; u < kMaxSlots / kTile; u++) {
+        const uint32_t sl = s0 + u;
+        if (u < per && sl < cfg.nslots) {
+            const uint32_t nt = (uint32_t)packed[u];
+            tile_start[sl] = base;
+            slots[sl].tile0 = base;
             // an expand step reads its tiles' frontier nodes from the index
-            if (nt[u] && (B.step == EXPAND_MID || B.step == EXPAND_LAST_COUNT)) B.io.tile_node = cfg.tile_node + 2 * base;
-            base += nt[u];
+            if (packed[u] & kSchedIndexed) slots[sl].io.tile_node = cfg.tile_node + 2 * base;
+            base += nt;
         }
     }
     if (tid == 0) {
-        cfg.tile_start[cfg.nslots] = total;
-        cfg.ctl->total_tiles = total;
-        cfg.ctl->running = running;
+        tile_start[cfg.nslots] = total;
+        gp(cfg.ctl)->total_tiles = total;
+        gp(cfg.ctl)->running = running;
+        gp(cfg.ctl)->n_admit = 0;     // every workgroup has read it
+        gp(cfg.ctl)->sched_done = 0;  // for the next launch
     }
 }
 
@@ -1710,7 +1775,7 @@ hipError_t launch_root(const RootArgs &a, uint32_t grid, hipStream_t s)
 hipError_t launch_batch_schedule(const BatchCfg &cfg, hipStream_t s)
 {
     if (cfg.nslots == 0 || cfg.nslots > kMaxSlots) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_batch_schedule, dim3(1), dim3(kTile), 0, s, cfg);
+    hipLaunchKernelGGL(k_batch_schedule, dim3((cfg.nslots + kWaves - 1) / kWaves), dim3(kTile), 0, s, cfg);
     return hipGetLastError();
 }
 

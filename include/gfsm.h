@@ -231,8 +231,9 @@ gfsm_status gfsm_match_noalloc(gfsm_ctx *ctx, const gfsm_graph *data, const gfsm
 void        gfsm_row_hash(const uint32_t *row, uint32_t width, uint64_t h[2]);
 
 /* ---- batching (no counterpart): n queries share every step.  Each step is
- * one scheduler launch plus one persistent step launch whose device work
- * queue holds the (query, tile) items of every query in flight; runs of steps
+ * a scheduler launch (one wave per query slot), a tile-index launch and one
+ * persistent step launch whose device work queue holds the (query, tile)
+ * items of every query in flight; runs of steps
  * are captured in a hipGraph and replayed.  opts holds nopts entries: 1 (the
  * same options for every query) or n (one per query).  Count and digest modes
  * run batched; queries asking for rows run on the single-query path.

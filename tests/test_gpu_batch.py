@@ -61,6 +61,33 @@ def test_batch_few_slots_continuous_admission(ctx, er_vlabel):
         ctx.set_tuning(batch_slots=saved["batch_slots"])
 
 
+def test_batch_scheduler_workgroup_counts(ctx, er_vlabel):
+    """The scheduler runs one wave per slot, four slots per workgroup, and the
+    last workgroup to finish lays the tiles out: slot counts that leave the
+    last workgroup partly empty (257: 65 workgroups, one slot in the last)
+    and the maximum (1024: 256 workgroups), batched and served, give the
+    single-query results."""
+    data = er_vlabel
+    qs = _queries(data, [3, 4, 5, 6, 2, 7] * 8, seed=1700)
+    ref = [ctx.match(data, q, digest=True) for q in qs]
+    want = [(r.count, r.digest, r.status) for r in ref]
+    saved = ctx.tuning()
+    try:
+        for slots in (257, 1024):
+            ctx.set_tuning(batch_slots=slots)
+            ms = ctx.match_batch(data, qs, digest=True)
+            assert [(m.count, m.digest, m.status) for m in ms] == want, slots
+    finally:
+        ctx.set_tuning(batch_slots=saved["batch_slots"])
+    for inflight in (257, 1024):
+        srv = G.Server(ctx, data, max_inflight=inflight, digest=True)
+        try:
+            got = [srv.wait(t) for t in [srv.submit(q) for q in qs]]
+        finally:
+            srv.stop()
+        assert [(m.count, m.digest, m.status) for m in got] == want, inflight
+
+
 @pytest.mark.parametrize("items,pool", [(1, 1024), (2, 4096), (4, 8192), (8, 14336), (8, 64)])
 def test_batch_shapes(ctx, er_vlabel, items, pool):
     """Every tile shape and staging pool (a tiny pool forces unstaged checks)."""

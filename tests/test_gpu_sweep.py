@@ -4,8 +4,8 @@ sizes and query shapes: every device path against the CPU oracle.
 Per case: count only (the lean last-depth kernel where the last depth has
 no checks), count and digest under each tile size (1, 2, 4 and 8 candidates
 per thread), sorted rows under a tight level budget (chunked expansion), the
-allocation-free entry point, and the batched engine over all queries of the
-same data graph."""
+allocation-free entry point, and the batched engine (256 slots and 3 slots)
+and a server (5 in flight) over all queries of the same data graph."""
 import numpy as np
 import pytest
 
@@ -18,12 +18,12 @@ TILES = (256, 512, 1024, 2048)
 
 
 def cases():
-    """16 data graphs with 3 queries each.  R-MAT graphs (hubs) run in the
+    """48 data graphs with 3 queries each.  R-MAT graphs (hubs) run in the
     labelled mode with at least 5 vertex labels and queries of at most 5
     vertices, so that counts stay within what the oracle enumerates in
     seconds; ER graphs run in both modes with queries of up to 6 vertices."""
     out = []
-    for seed in range(16):
+    for seed in range(48):
         rmat = seed % 4 == 1
         mode = G.MODE_VLABEL if (seed % 2 or rmat) else G.MODE_REF
         nvl = 5 + seed % 3 if rmat else 1 + seed % 4
@@ -31,7 +31,7 @@ def cases():
         if rmat:
             data = ("rmat", dict(scale=11, edge_factor=6, nvlabels=nvl, nelabels=nel, seed=700 + seed, mode=mode))
         else:
-            data = ("er", dict(n=1500 + 100 * seed, avg_deg=4.0 + seed % 3, nvlabels=nvl, nelabels=nel,
+            data = ("er", dict(n=1500 + 100 * (seed % 16), avg_deg=4.0 + seed % 3, nvlabels=nvl, nelabels=nel,
                                seed=700 + seed, mode=mode))
         qs = [(3 + (seed + k) % (3 if rmat else 4), G.WALK_DENSE if (seed + k) % 2 else G.WALK_SPARSE,
                7000 + 10 * seed + k) for k in range(3)]
@@ -70,7 +70,19 @@ def test_sweep_every_path_vs_oracle(ctx, seed, data_spec, qspecs):
             assert_same(r, oracle_solve(data, q, mode, rows=True))
         rc, n, dg = ctx.match_noalloc(data, q, digest=True)
         assert (rc, n, dg) == (0, ref.count, ref.digest), (seed, i)
+    want = [(ref.count, ref.digest) for ref in refs]
     batched = ctx.match_batch(data, queries, digest=True)
-    for i, (b, ref) in enumerate(zip(batched, refs)):
-        assert (b.count, b.digest) == (ref.count, ref.digest), (seed, i)
+    assert [(b.count, b.digest) for b in batched] == want, seed
+    saved = ctx.tuning()
+    try:
+        ctx.set_tuning(batch_slots=3)
+        assert [(b.count, b.digest) for b in ctx.match_batch(data, queries, digest=True)] == want, seed
+    finally:
+        ctx.set_tuning(batch_slots=saved["batch_slots"])
+    srv = G.Server(ctx, data, max_inflight=5, digest=True)
+    try:
+        served = [srv.wait(t) for t in [srv.submit(q) for q in queries]]
+    finally:
+        srv.stop()
+    assert [(m.count, m.digest) for m in served] == want, seed
     assert nonzero >= 1, seed

@@ -372,15 +372,22 @@ class Match:
         dg = (C.c_uint64 * 2)()
         _check(L.gfsm_result_digest(handle, dg))
         self.digest = (int(dg[0]), int(dg[1]))
-        st = Stats()
-        _check(L.gfsm_result_stats(handle, C.byref(st)))
-        self.stats = st.as_dict()
+        self._stats = None  # read from the result on first use
         self.status = L.gfsm_result_status(handle)
         p = C.POINTER(C.c_uint32)()
         w, nr = C.c_uint32(), C.c_uint64()
         _check(L.gfsm_result_rows(handle, C.byref(p), C.byref(w), C.byref(nr)))
         self.width = w.value
         self.rows = _view(p, nr.value * w.value).reshape(nr.value, w.value) if nr.value else None
+
+    @property
+    def stats(self):
+        """Kernel statistics of the match (a dict), read on first use."""
+        if self._stats is None:
+            st = Stats()
+            _check(lib().gfsm_result_stats(self._h, C.byref(st)))
+            self._stats = st.as_dict()
+        return self._stats
 
     def write_fs(self, path):
         _check(lib().gfsm_result_write_fs(self._h, str(path).encode()), "write_fs")

@@ -48,7 +48,7 @@ DEV gptr_t<T> gp(T *p)
 constexpr uint32_t kWaves = kTile / 64;
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));  // one 16-byte store
 constexpr uint32_t kNodeLk = 4;  // node-setup lookups a thread keeps in flight
-constexpr uint32_t kStageLd = 4; // row words a thread loads before storing them to LDS
+constexpr uint32_t kStageLd = 8; // row words a thread loads before storing them to LDS
 
 // ---------------------------------------------------------------------------
 // wave64 primitives
